@@ -1,0 +1,141 @@
+/*
+ * lbm_oracle.h -- CPU oracle for the Inamuro two-phase D3Q15 time step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker against which the
+ * HIP path is compared.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product library
+ * (lbm_amd/liblbm_hip.so) never links or calls it.
+ *
+ * It restates the reference CPU solver (CPU/Inamuro.cpp, CPU/Inamuro.hpp,
+ * CPU/common.hpp) operation for operation, including the association order of
+ * every floating-point expression and the x-outer / y / z-inner order of every
+ * global sum, so that gcc -O3 (no FMA contraction) reproduces the reference's
+ * results bit for bit.  Each function cites the reference lines it follows.
+ *
+ * Layouts at this API boundary are canonical: macro fields are [z][y][x]
+ * (x fastest, no ghost layers), distributions are [q][z][y][x].
+ */
+#ifndef LBM_ORACLE_H
+#define LBM_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct orc_params {
+    int lx, ly, lz;
+    double rho_L, rho_G;
+    double mu_L, mu_G;
+    double tauf, taug;
+    double k_f, k_g;
+    double T, a, b;
+    double fei_max, fei_min;
+    double fei_L, fei_G;
+    double DD;
+} orc_params;
+
+/* Mirrors the CPU PressureSolveDiagnostics (CPU/Inamuro.hpp:76-83) plus the
+ * GPU PoissonStepDiagnostics fields (GPU/InamuroCUDA.hpp:19-43) that the
+ * restated dual residual fills. */
+typedef struct orc_poisson_diag {
+    int32_t pressure_solve_call_count;
+    int32_t hit_max_iterations;
+    uint64_t prev_iteration_count;
+    uint64_t iteration_count;
+    double relative_error;
+    double pressure_l1_delta;
+    double pressure_l1_norm;
+    double divergence_mean;
+    double pressure_gauge_target;
+    double pressure_gauge_max_shift;
+    double fixed_point_h_l1, fixed_point_h_scale, fixed_point_h_relative;
+    double fixed_point_p_l1, fixed_point_p_scale, fixed_point_p_relative;
+    double fixed_point_relative;
+    int32_t fixed_point_evaluated;
+    int32_t converged;
+} orc_poisson_diag;
+
+/* Array identifiers for orc_get / orc_set. */
+enum {
+    ORC_FEI = 0, ORC_RHO, ORC_U, ORC_V, ORC_W, ORC_P,
+    ORC_FEI_X, ORC_FEI_Y, ORC_FEI_Z,
+    ORC_RHO_X, ORC_RHO_Y, ORC_RHO_Z,
+    ORC_U_X, ORC_U_Y, ORC_U_Z,
+    ORC_V_X, ORC_V_Y, ORC_V_Z,
+    ORC_W_X, ORC_W_Y, ORC_W_Z,
+    ORC_FEI_LAP, ORC_U_LAP, ORC_V_LAP, ORC_W_LAP,
+    ORC_P_PREV,
+    ORC_NUM_SCALAR_ARRAYS,
+    ORC_F = 100, ORC_G, ORC_H
+};
+
+/* Stage identifiers for orc_stage (single reference functions). */
+enum {
+    ORC_STAGE_GRADIENTS = 0,  /* the 5 firstord + 4 secondord calls of collision() */
+    ORC_STAGE_COLLISION,      /* full Inamuro::collision() (stencils + BGK)        */
+    ORC_STAGE_STREAM_FG,      /* stream(ff), stream(gg)                             */
+    ORC_STAGE_BC_FG,          /* applyBoundaryConditions(ff), (gg)                  */
+    ORC_STAGE_MACRO,          /* getMacro()                                         */
+    ORC_STAGE_POISSON,        /* solvePressurePoisson()                             */
+    ORC_STAGE_CORRECT,        /* correct_uvw()                                      */
+    ORC_STAGE_UPDATE_HH,      /* update_hh()                                        */
+    ORC_STAGE_COLLISION_P,    /* one collision_p()                                  */
+    ORC_STAGE_STREAM_H,       /* stream(hh)                                         */
+    ORC_STAGE_BC_H,           /* applyBoundaryConditions(hh)                        */
+    ORC_STAGE_GETP            /* getp()                                             */
+};
+
+typedef struct orc_sim orc_sim;
+
+/* Allocates arrays and applies Inamuro::initializeArrays (CPU/Inamuro.cpp:511-548):
+ * f=g=h=0, rho=rho_G, fei=fei_min, u=v=w=p=0. Returns NULL on bad sizes. */
+orc_sim* orc_create(const orc_params* params);
+void orc_destroy(orc_sim* s);
+const char* orc_last_error(const orc_sim* s);
+
+/* Worker threads for per-cell loops (global sums always stay sequential, so
+ * results are bit-identical for any thread count). */
+void orc_set_threads(orc_sim* s, int nthreads);
+
+/* Poisson loop settings.  The reference hard-codes check_interval=100 and
+ * tol=1e-3 (CPU/Inamuro.cpp:670-671); iter_cap=0 is the unbounded contract,
+ * a nonzero cap is a test watchdog only.  dual=1 adds the restated GPU-only
+ * kinetic fixed-point residual to the exit test (DUAL_RESIDUAL_CONTRACT.md). */
+int orc_set_poisson(orc_sim* s, uint64_t check_interval, double tol,
+                    uint64_t iter_cap, int dual);
+
+/* Initial conditions. */
+void orc_init_two_droplets_sharp(orc_sim* s);          /* CPU/Inamuro.cpp:550-587 */
+/* GPU/Inamuro.cpp:499-533 place_droplet, generalised to a velocity vector:
+ * profile 0 = sharp, 1 = tanh. */
+void orc_place_droplet(orc_sim* s, double cx, double cy, double cz,
+                       double vx, double vy, double vz,
+                       int profile, double interface_width);
+
+int orc_get(const orc_sim* s, int array_id, double* out);
+int orc_set(orc_sim* s, int array_id, const double* in);
+
+/* One full Inamuro::performTimeStep (CPU/Inamuro.cpp:347-363). 0 = ok,
+ * negative = error (message in orc_last_error). */
+int orc_step(orc_sim* s);
+int orc_stage(orc_sim* s, int stage);
+int orc_is_first_step(const orc_sim* s);
+void orc_set_first_step(orc_sim* s, int first);
+void orc_poisson_diag_get(const orc_sim* s, orc_poisson_diag* out);
+
+/* Closed-form helpers exposed for unit tests. */
+double orc_eos(const orc_params* p, double fei);                 /* CPU/Inamuro.cpp:843-847 */
+double orc_rho_of_phi(const orc_params* p, double fei);           /* CPU/Inamuro.cpp:313-327 */
+
+/* Timing helper for the CPU baseline: runs k bare Poisson iterations
+ * (collision_p, stream, BC, getp) with no convergence test. */
+void orc_poisson_iterations(orc_sim* s, int k);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif
