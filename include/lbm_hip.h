@@ -1,0 +1,187 @@
+/*
+ * lbm_hip.h -- C ABI of the MI355X (gfx950) Inamuro two-phase D3Q15 solver.
+ *
+ * This is the drop-in boundary for the reference's GPU backend class
+ * InamuroCUDA (GPU/InamuroCUDA.hpp:16-248): every entry point below replaces
+ * one member of that class, cited next to it.  Plain pointers and sizes only;
+ * no exception crosses the ABI.  Every function returns LBM_OK (0) or a
+ * negative status, and lbm_last_error() returns the message the reference
+ * would have thrown (same text as CPU/Inamuro.cpp and GPU/InamuroCUDA.cu).
+ *
+ * Host layouts (caller-owned, copies are synchronous):
+ *   macro field   double[lz][ly][lx]        (x fastest, no ghost layers)
+ *   distribution  double[15][lz][ly][lx]    (q outer)
+ * The reference keeps [x][y][z+1] nested vectors (CPU/Inamuro.hpp:21-28) and
+ * a cell*15+q device layout (GPU/InamuroCUDA.hpp:236-247); the ABI uses the
+ * canonical layout above and the device uses SoA [q][z][y][x].
+ *
+ * Threading: one context per host thread; calls on one context are
+ * serialised by the caller.  A context owns one HIP stream on the device that
+ * is current when lbm_create is called.  After a numeric error the context's
+ * fields are undefined until the next lbm_upload.
+ */
+#ifndef LBM_HIP_H
+#define LBM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LBM_ABI_VERSION 1
+
+/* Status codes. */
+#define LBM_OK            0
+#define LBM_ERR_ARG      -1   /* invalid argument (reference: std::invalid_argument) */
+#define LBM_ERR_HIP      -2   /* HIP runtime failure (reference: "CUDA failure")     */
+#define LBM_ERR_NUMERIC  -3   /* non-finite source / gauge / residual (runtime_error) */
+#define LBM_ERR_OVERFLOW -4   /* counter overflow (reference: std::overflow_error)   */
+#define LBM_ERR_STATE    -5   /* call out of order, e.g. step before upload           */
+#define LBM_ERR_NOMEM    -6   /* device allocation failed                             */
+
+/* Arithmetic of the time step. */
+#define LBM_PRECISION_FP64  0  /* plain FP64, reference op order, no FMA contraction */
+#define LBM_PRECISION_MIXED 1  /* FP32 f/g/h deviations + FP32 collision; FP64 p,
+                                   Poisson reductions, gauge and mass sums          */
+
+/* The 14 positional parameters of params.in (CPU/Inamuro.cpp:37-53,
+ * CPU/Inamuro.hpp:39-68).  `period`, max_time_steps and output_frequency
+ * belong to the driver, not to the solver. */
+typedef struct lbm_params_s {
+    int32_t lx, ly, lz;
+    double rho_L, rho_G;
+    double mu_L, mu_G;
+    double tauf, taug;
+    double k_f, k_g;
+    double T, a, b;
+    double fei_max, fei_min;
+    double fei_L, fei_G;
+    double DD;
+} lbm_params_t;
+
+/* Mirrors InamuroCUDA::PoissonStepDiagnostics (GPU/InamuroCUDA.hpp:19-43) plus
+ * the CPU solver's PressureSolveDiagnostics counters (CPU/Inamuro.hpp:76-83). */
+typedef struct lbm_poisson_diag_s {
+    uint64_t iterations;
+    uint64_t prev_iterations;
+    int32_t pressure_solve_call_count;
+    int32_t hit_max_iterations;
+    double relative_error;
+    double pressure_l1_delta;
+    double pressure_l1_norm;
+    double fixed_point_h_l1;
+    double fixed_point_h_scale;
+    double fixed_point_h_relative;
+    double fixed_point_p_l1;
+    double fixed_point_p_scale;
+    double fixed_point_p_relative;
+    double fixed_point_relative;
+    double source_mean;            /* -mean(div)/3, GPU/InamuroCUDA.cu:1891          */
+    double divergence_mean;        /* mean(u_x+v_y+w_z) removed from u_x              */
+    double pressure_gauge_target;
+    double pressure_gauge_max_shift;
+    int32_t fixed_point_evaluated;
+    int32_t pressure_converged;
+    int32_t fixed_point_converged;
+    int32_t dual_residual_enabled;
+    int32_t converged;
+    int32_t reserved;
+} lbm_poisson_diag_t;
+
+/* Event-timed totals (reference: PerformanceMetrics, GPU/InamuroCUDA.hpp:115-132). */
+typedef struct lbm_perf_s {
+    int64_t steps;
+    uint64_t poisson_iterations;
+    uint64_t poisson_checks;
+    double collide_ms;          /* stencils + f/g collide-and-pull              */
+    double macro_ms;            /* source reduction + macro pull                */
+    double poisson_ms;          /* Poisson iteration kernels only               */
+    double check_ms;            /* gauge / residual / dual kernels + read-back  */
+    double correct_ms;          /* velocity correction                          */
+    uint64_t poisson_iter_launches;
+} lbm_perf_t;
+
+/* InamuroCUDA(const Inamuro&) (GPU/InamuroCUDA.hpp:46): allocates device state
+ * for the grid in *params.  Fields start as Inamuro::initializeArrays leaves
+ * them (CPU/Inamuro.cpp:511-548) until lbm_upload is called. */
+int lbm_create(const lbm_params_t* params, int precision, void** out_ctx);
+
+/* ~InamuroCUDA (GPU/InamuroCUDA.cu:1463-1467). */
+int lbm_destroy(void* ctx);
+
+/* Message of the last failure on ctx (or of the last failed lbm_create when
+ * ctx is NULL). */
+const char* lbm_last_error(const void* ctx);
+
+/* initFromCPU (GPU/InamuroCUDA.cu:1628-1673): uploads the macro fields
+ * fei, rho, u, v, w, p (any may be NULL to keep the current value); resets
+ * h to 0, the first-step flag (f=feq, g=geq on the next step) and the
+ * persistent p_prev to 0, as a freshly constructed reference solver has. */
+int lbm_upload(void* ctx, const double* fei, const double* rho, const double* u,
+               const double* v, const double* w, const double* p);
+
+/* Inamuro::initializeArrays + initializeDroplets of the CPU tree
+ * (CPU/Inamuro.cpp:511-587): background fei_min / rho_G / zero velocity and
+ * pressure, then two sharp droplets of diameter DD at x=0, z=lz/2 +- DD with
+ * w = -+0.035.  Computed on the host with libm, then uploaded as lbm_upload. */
+int lbm_init_reference(void* ctx);
+
+/* place_droplet of the GPU-tree constructor (GPU/Inamuro.cpp:499-533):
+ * radius DD/2 centred at (cx,cy,cz); profile 0 = sharp, 1 = tanh with
+ * interface_width (<= 0 selects 2.0); a cell is overwritten only where the
+ * droplet's phi exceeds the current phi, with rho blended linearly and the
+ * velocity scaled by the liquid fraction (the reference sets w only; vx, vy
+ * extend it the same way).  Applies to the current fields and then behaves
+ * as lbm_upload of the result.  Call after lbm_reset_fields to start from
+ * the background state. */
+int lbm_place_droplet(void* ctx, double cx, double cy, double cz, double vx, double vy,
+                      double vz, int profile, double interface_width);
+
+/* Inamuro::initializeArrays (CPU/Inamuro.cpp:511-548) followed by the
+ * lbm_upload reset. */
+int lbm_reset_fields(void* ctx);
+
+/* initFromCPU for the distributions (GPU/InamuroCUDA.cu:1631-1648): f, g, h
+ * as the reference stores them after a step (post stream + boundary).
+ * first_step=1 keeps the f=feq initialisation of the next collision. */
+int lbm_upload_dists(void* ctx, const double* f, const double* g, const double* h,
+                     int first_step);
+
+/* setPoissonConvergence / setPoissonIterationLimit / setUsePoissonDualResidual
+ * (GPU/InamuroCUDA.cu:2573-2608).  iteration_limit = 0 is the unbounded
+ * contract; a nonzero limit must be a multiple of check_interval. */
+int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance,
+                    uint64_t iteration_limit, int dual_residual);
+
+/* performTimeStepGPU (GPU/InamuroCUDA.cu:2372-2442), n times. */
+int lbm_step(void* ctx, int32_t n);
+
+/* downloadFieldsToCPU (GPU/InamuroCUDA.cu:1675-1701). NULL skips a field. */
+int lbm_download(void* ctx, double* fei, double* rho, double* u, double* v,
+                 double* w, double* p);
+
+/* downloadHH (GPU/InamuroCUDA.cu:1703-1707), extended to f and g: the
+ * distributions as the reference holds them after a step. NULL skips. */
+int lbm_download_dists(void* ctx, double* f, double* g, double* h);
+
+/* getLastPoissonDiagnostics (GPU/InamuroCUDA.cu:1714-1717). */
+int lbm_poisson_diag(const void* ctx, lbm_poisson_diag_t* out);
+
+/* synchronize (GPU/InamuroCUDA.cu:1709-1712). */
+int lbm_synchronize(void* ctx);
+
+/* Event timing of each phase (GPU/InamuroCUDA.cu:2380-2437); off by default. */
+int lbm_set_timing(void* ctx, int enabled);
+int lbm_perf_get(const void* ctx, lbm_perf_t* out);
+int lbm_perf_reset(void* ctx);
+
+/* Device global sums in FP64: sum of phi (mass) and of |u| over the grid. */
+int lbm_mass(void* ctx, double* sum_phi);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* LBM_HIP_H */

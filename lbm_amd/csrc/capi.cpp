@@ -1,0 +1,704 @@
+// capi.cpp -- C ABI (include/lbm_hip.h) and host orchestration of the time
+// step.  Mirrors InamuroCUDA::performTimeStepGPU (GPU/InamuroCUDA.cu:2372-2442)
+// and doPressurePoisson (GPU/InamuroCUDA.cu:1816-2349) with the CPU solver's
+// semantics (CPU/Inamuro.cpp:347-363, 624-724):
+//
+//   collide (stencils + BGK + pull of f,g)  -> source/gauge-target reduction
+//   -> macro (pull of f,g, rho map, tau_h, projected source)
+//   -> Poisson: segments of `check_interval` fused iterations, then one
+//      on-device check (gauge, relative change, optional dual residual) and a
+//      single device-to-host read of a small state struct
+//   -> velocity correction.  h = E*p is folded into the next step's first
+//      iteration instead of being written.
+//
+// No exception crosses the ABI; messages are the reference's.
+#include "../../include/lbm_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace {
+
+using lbm::DevState;
+using lbm::Grid;
+using lbm::Phys;
+
+std::string g_create_error;
+
+struct HipError {
+    hipError_t code;
+    const char* what;
+};
+
+#define HIPCHK(call)                                                \
+    do {                                                            \
+        hipError_t e__ = (call);                                    \
+        if (e__ != hipSuccess) throw HipError{e__, #call};          \
+    } while (0)
+
+struct ApiError {
+    int code;
+    std::string msg;
+};
+
+enum class FG { First, Post, Local };
+enum class HState { Explicit, Equilibrium };
+
+struct Ctx {
+    lbm_params_t P{};
+    int precision = LBM_PRECISION_FP64;
+    Grid g{};
+    Phys ph{};
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // FP64 device buffers.
+    double* f[2] = {nullptr, nullptr};
+    double* gd[2] = {nullptr, nullptr};
+    double* h[2] = {nullptr, nullptr};
+    double* p[2] = {nullptr, nullptr};
+    double *fei = nullptr, *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+    double *tau = nullptr, *src = nullptr, *ux = nullptr, *vy = nullptr, *wz = nullptr;
+    double* p_prev = nullptr;
+    double* partials = nullptr;
+    double* scratch = nullptr;  // one distribution array, for downloads
+    DevState* dst = nullptr;
+    DevState* hst = nullptr;    // pinned mirror
+
+    int fcur = 0, hcur = 0, pcur = 0;
+    FG fg = FG::First;
+    HState hs = HState::Explicit;
+
+    int check_interval = 100;
+    double tol = 1.0e-3;
+    uint64_t iter_limit = 0;
+    int dual = 0;
+
+    lbm_poisson_diag_t diag{};
+    uint64_t last_iterations = 0;
+    int32_t solve_calls = 0;
+
+    bool timing = false;
+    lbm_perf_t perf{};
+    hipEvent_t ev[10] = {};
+
+    std::string err;
+
+    size_t n() const { return static_cast<size_t>(g.n); }
+};
+
+void free_all(Ctx& c)
+{
+    (void)hipSetDevice(c.device);
+    auto F = [](double*& ptr) {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+    };
+    for (int i = 0; i < 2; ++i) {
+        F(c.f[i]);
+        F(c.gd[i]);
+        F(c.h[i]);
+        F(c.p[i]);
+    }
+    F(c.fei); F(c.rho); F(c.u); F(c.v); F(c.w);
+    F(c.tau); F(c.src); F(c.ux); F(c.vy); F(c.wz);
+    F(c.p_prev); F(c.partials); F(c.scratch);
+    if (c.dst) (void)hipFree(c.dst);
+    c.dst = nullptr;
+    if (c.hst) (void)hipHostFree(c.hst);
+    c.hst = nullptr;
+    for (auto& e : c.ev) {
+        if (e) (void)hipEventDestroy(e);
+        e = nullptr;
+    }
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    c.stream = nullptr;
+}
+
+double* dalloc(size_t count)
+{
+    void* ptr = nullptr;
+    const hipError_t e = hipMalloc(&ptr, count * sizeof(double));
+    if (e != hipSuccess) throw ApiError{LBM_ERR_NOMEM, "device allocation failed"};
+    return static_cast<double*>(ptr);
+}
+
+void fill(Ctx& c, double* dst, double value, size_t count)
+{
+    std::vector<double> tmp(count, value);
+    HIPCHK(hipMemcpyAsync(dst, tmp.data(), count * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    HIPCHK(hipStreamSynchronize(c.stream));
+}
+
+void reset_state(Ctx& c)
+{
+    // Inamuro::initializeArrays, CPU/Inamuro.cpp:511-548.
+    const size_t n = c.n();
+    HIPCHK(hipMemsetAsync(c.h[0], 0, lbm::Q * n * sizeof(double), c.stream));
+    HIPCHK(hipMemsetAsync(c.h[1], 0, lbm::Q * n * sizeof(double), c.stream));
+    HIPCHK(hipMemsetAsync(c.f[0], 0, lbm::Q * n * sizeof(double), c.stream));
+    HIPCHK(hipMemsetAsync(c.gd[0], 0, lbm::Q * n * sizeof(double), c.stream));
+    HIPCHK(hipMemsetAsync(c.p_prev, 0, n * sizeof(double), c.stream));
+    HIPCHK(hipMemsetAsync(c.dst, 0, sizeof(DevState), c.stream));
+    c.fcur = c.hcur = c.pcur = 0;
+    c.fg = FG::First;
+    c.hs = HState::Explicit;
+    c.last_iterations = 0;
+    c.solve_calls = 0;
+    c.diag = lbm_poisson_diag_t{};
+}
+
+void check_params(const lbm_params_t& P)
+{
+    if (P.lx < 3 || P.ly < 1 || P.lz < 1)
+        throw ApiError{LBM_ERR_ARG, "grid needs lx >= 3 (x-slip mirror) and ly, lz >= 1"};
+    const int64_t n = static_cast<int64_t>(P.lx) * P.ly * P.lz;
+    if (n > (int64_t(1) << 31) - 1)
+        throw ApiError{LBM_ERR_ARG, "grid exceeds 2^31-1 cells"};
+    if (!(P.rho_L != P.rho_G) || !(P.fei_L != P.fei_G))
+        throw ApiError{LBM_ERR_ARG, "rho_L/rho_G and fei_L/fei_G must differ"};
+}
+
+Ctx* create(const lbm_params_t& P, int precision)
+{
+    check_params(P);
+    if (precision != LBM_PRECISION_FP64 && precision != LBM_PRECISION_MIXED)
+        throw ApiError{LBM_ERR_ARG, "unknown precision"};
+    if (precision != LBM_PRECISION_FP64)
+        throw ApiError{LBM_ERR_ARG, "precision not available in this build"};
+    Ctx* c = new Ctx();
+    try {
+        c->P = P;
+        c->precision = precision;
+        HIPCHK(hipGetDevice(&c->device));
+        c->g = Grid{P.lx, P.ly, P.lz, static_cast<int64_t>(P.lx) * P.ly * P.lz};
+        c->ph = Phys{P.rho_L, P.rho_G, P.mu_L, P.mu_G, P.tauf, P.taug, P.k_f, P.k_g, P.T, P.a, P.b,
+                     P.fei_L, P.fei_G,
+                     (P.mu_L - P.mu_G) / (P.rho_L - P.rho_G),
+                     (P.fei_L + P.fei_G) / 2.0,
+                     static_cast<double>(P.lx) * P.ly * P.lz};
+        HIPCHK(hipStreamCreate(&c->stream));
+        const size_t n = c->n();
+        for (int i = 0; i < 2; ++i) {
+            c->f[i] = dalloc(lbm::Q * n);
+            c->gd[i] = dalloc(lbm::Q * n);
+            c->h[i] = dalloc(lbm::Q * n);
+            c->p[i] = dalloc(n);
+        }
+        c->fei = dalloc(n); c->rho = dalloc(n); c->u = dalloc(n); c->v = dalloc(n); c->w = dalloc(n);
+        c->tau = dalloc(n); c->src = dalloc(n); c->ux = dalloc(n); c->vy = dalloc(n); c->wz = dalloc(n);
+        c->p_prev = dalloc(n);
+        const size_t nparts = std::max<size_t>(2 * static_cast<size_t>(lbm::grid_blocks(c->g.n)),
+                                               9 * static_cast<size_t>(lbm::RED_BLOCKS));
+        c->partials = dalloc(nparts);
+        HIPCHK(hipMalloc(&c->dst, sizeof(DevState)));
+        HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hst), sizeof(DevState), hipHostMallocDefault));
+        for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
+        fill(*c, c->rho, P.rho_G, n);
+        fill(*c, c->fei, P.fei_min, n);
+        HIPCHK(hipMemsetAsync(c->u, 0, n * sizeof(double), c->stream));
+        HIPCHK(hipMemsetAsync(c->v, 0, n * sizeof(double), c->stream));
+        HIPCHK(hipMemsetAsync(c->w, 0, n * sizeof(double), c->stream));
+        HIPCHK(hipMemsetAsync(c->p[0], 0, n * sizeof(double), c->stream));
+        reset_state(*c);
+        HIPCHK(hipStreamSynchronize(c->stream));
+    } catch (...) {
+        free_all(*c);
+        delete c;
+        throw;
+    }
+    return c;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b)
+{
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+void record(Ctx& c, int i)
+{
+    if (c.timing) HIPCHK(hipEventRecord(c.ev[i], c.stream));
+}
+
+const char* status_message(int32_t st)
+{
+    switch (st) {
+    case lbm::ST_SOURCE_NONFINITE: return "Poisson source/gauge contains NaN/Inf";
+    case lbm::ST_GAUGE_NONFINITE: return "pressure gauge is NaN/Inf";
+    case lbm::ST_PRESSURE_NONFINITE: return "Poisson pressure/residual contains NaN/Inf";
+    case lbm::ST_RESIDUAL_NONFINITE: return "Poisson residual is NaN/Inf";
+    case lbm::ST_FP_NONFINITE: return "kinetic fixed-point residual contains NaN/Inf";
+    case lbm::ST_FP_REL_NONFINITE: return "normalized kinetic fixed-point residual contains NaN/Inf";
+    default: return "unknown device status";
+    }
+}
+
+// One performTimeStep.
+void step_one(Ctx& c)
+{
+    hipStream_t s = c.stream;
+    const Grid& g = c.g;
+    // 1. collision fused with the pull of the streamed f and g.
+    record(c, 0);
+    const int mode = c.fg == FG::First ? lbm::FG_FIRST : (c.fg == FG::Post ? lbm::FG_PULL : lbm::FG_LOCAL);
+    const int nxt = 1 - c.fcur;
+    lbm::f64::collide(s, mode, g, c.ph, c.fei, c.rho, c.u, c.v, c.w, c.p[c.pcur],
+                      c.f[c.fcur], c.gd[c.fcur], c.f[nxt], c.gd[nxt], c.ux, c.vy, c.wz, c.partials);
+    HIPCHK(hipGetLastError());
+    c.fcur = nxt;
+    c.fg = FG::Post;
+    record(c, 1);
+    // 2. source statistics, then macro fields from the pulled populations.
+    lbm::f64::source_finalize(s, g, c.ph, c.partials, lbm::grid_blocks(g.n), c.dst);
+    lbm::f64::macro(s, 1, g, c.ph, c.f[c.fcur], c.gd[c.fcur], c.ux, c.vy, c.wz, c.dst,
+                    c.fei, c.rho, c.u, c.v, c.w, c.tau, c.src);
+    HIPCHK(hipGetLastError());
+    record(c, 2);
+
+    // 3. Poisson solve.
+    ++c.solve_calls;
+    lbm_poisson_diag_t& d = c.diag;
+    const uint64_t prev = c.last_iterations;
+    d = lbm_poisson_diag_t{};
+    d.prev_iterations = prev;
+    d.pressure_solve_call_count = c.solve_calls;
+    d.dual_residual_enabled = c.dual;
+    const double nan = std::numeric_limits<double>::quiet_NaN();
+    d.fixed_point_h_l1 = d.fixed_point_h_scale = d.fixed_point_h_relative = nan;
+    d.fixed_point_p_l1 = d.fixed_point_p_scale = d.fixed_point_p_relative = nan;
+    d.fixed_point_relative = nan;
+
+    int hmode = c.hs == HState::Equilibrium ? lbm::H_EQUILIBRIUM : lbm::H_EXPLICIT;
+    uint64_t iter = 0;
+    bool first_segment = true;
+    for (;;) {
+        record(c, 4);
+        for (int k = 0; k < c.check_interval; ++k) {
+            if (iter == std::numeric_limits<uint64_t>::max())
+                throw ApiError{LBM_ERR_OVERFLOW, "Poisson iteration counter overflow"};
+            ++iter;
+            lbm::f64::poisson_iter(s, hmode, g, c.h[c.hcur], c.h[1 - c.hcur], c.p[c.pcur],
+                                   c.tau, c.src, c.dst);
+            c.hcur = 1 - c.hcur;
+            hmode = lbm::H_STREAMED;
+        }
+        HIPCHK(hipGetLastError());
+        record(c, 5);
+        lbm::f64::check(s, g, c.ph, c.p[c.pcur], c.p_prev, c.h[c.hcur], c.tau, c.src, c.dual, c.tol,
+                        c.partials, c.dst);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c.hst, c.dst, sizeof(DevState), hipMemcpyDeviceToHost, s));
+        record(c, 6);
+        HIPCHK(hipStreamSynchronize(s));
+        if (c.timing) {
+            if (first_segment) {
+                c.perf.collide_ms += elapsed(c.ev[0], c.ev[1]);
+                c.perf.macro_ms += elapsed(c.ev[1], c.ev[2]);
+            }
+            c.perf.poisson_ms += elapsed(c.ev[4], c.ev[5]);
+            c.perf.check_ms += elapsed(c.ev[5], c.ev[6]);
+        }
+        first_segment = false;
+        c.perf.poisson_iter_launches += static_cast<uint64_t>(c.check_interval);
+        c.perf.poisson_checks += 1;
+        const DevState& st = *c.hst;
+        d.iterations = iter;
+        d.relative_error = st.rel;
+        d.pressure_l1_delta = st.e1;
+        d.pressure_l1_norm = st.e2;
+        d.divergence_mean = st.div_mean;
+        d.source_mean = -st.div_mean / 3.0;
+        d.pressure_gauge_target = st.gauge_target;
+        d.pressure_gauge_max_shift = st.max_shift;
+        d.pressure_converged = st.pressure_converged;
+        if (c.dual) {
+            d.fixed_point_h_l1 = st.fp[0];
+            d.fixed_point_h_scale = st.fp[1] > st.fp[2] ? st.fp[1] : st.fp[2];
+            d.fixed_point_h_relative = st.fp_h_rel;
+            d.fixed_point_p_l1 = st.fp[3];
+            d.fixed_point_p_scale = st.fp[4] > st.fp[5] ? st.fp[4] : st.fp[5];
+            d.fixed_point_p_relative = st.fp_p_rel;
+            d.fixed_point_relative = st.fp_rel;
+            d.fixed_point_evaluated = st.fp_evaluated;
+            d.fixed_point_converged = st.fp_converged;
+        }
+        d.converged = st.converged;
+        if (st.status != lbm::ST_OK) throw ApiError{LBM_ERR_NUMERIC, status_message(st.status)};
+        if (st.converged) break;
+        if (c.iter_limit != 0 && iter >= c.iter_limit) {
+            d.hit_max_iterations = 1;
+            break;
+        }
+        hmode = lbm::H_GAUGED;
+    }
+    c.last_iterations = iter;
+    c.perf.poisson_iterations += iter;
+
+    // 4. velocity correction; h = E*p is implied for the next step.
+    record(c, 7);
+    lbm::f64::correct(s, g, c.p[c.pcur], c.rho, c.u, c.v, c.w);
+    HIPCHK(hipGetLastError());
+    record(c, 8);
+    c.hs = HState::Equilibrium;
+    if (c.timing) {
+        HIPCHK(hipEventSynchronize(c.ev[8]));
+        c.perf.correct_ms += elapsed(c.ev[7], c.ev[8]);
+    }
+    c.perf.steps += 1;
+}
+
+template <typename Fn>
+int guarded(void* ctx, Fn&& fn)
+{
+    if (!ctx) return LBM_ERR_ARG;
+    Ctx& c = *static_cast<Ctx*>(ctx);
+    try {
+        HIPCHK(hipSetDevice(c.device));
+        fn(c);
+        c.err.clear();
+        return LBM_OK;
+    } catch (const ApiError& e) {
+        c.err = e.msg;
+        return e.code;
+    } catch (const HipError& e) {
+        c.err = std::string("HIP failure: ") + hipGetErrorString(e.code) + " in " + e.what;
+        return LBM_ERR_HIP;
+    } catch (const std::bad_alloc&) {
+        c.err = "host allocation failed";
+        return LBM_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        c.err = e.what();
+        return LBM_ERR_ARG;
+    }
+}
+
+void upload_field(Ctx& c, double* dst, const double* src)
+{
+    if (src) HIPCHK(hipMemcpyAsync(dst, src, c.n() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+}
+
+void download_field(Ctx& c, double* dst, const double* src)
+{
+    if (dst) HIPCHK(hipMemcpyAsync(dst, src, c.n() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+}
+
+struct HostFields {
+    std::vector<double> fei, rho, u, v, w, p;
+};
+
+HostFields download_host(Ctx& c)
+{
+    HostFields h;
+    for (auto* a : {&h.fei, &h.rho, &h.u, &h.v, &h.w, &h.p}) a->resize(c.n());
+    download_field(c, h.fei.data(), c.fei);
+    download_field(c, h.rho.data(), c.rho);
+    download_field(c, h.u.data(), c.u);
+    download_field(c, h.v.data(), c.v);
+    download_field(c, h.w.data(), c.w);
+    download_field(c, h.p.data(), c.p[c.pcur]);
+    HIPCHK(hipStreamSynchronize(c.stream));
+    return h;
+}
+
+void upload_host(Ctx& c, const HostFields& h)
+{
+    reset_state(c);
+    upload_field(c, c.fei, h.fei.data());
+    upload_field(c, c.rho, h.rho.data());
+    upload_field(c, c.u, h.u.data());
+    upload_field(c, c.v, h.v.data());
+    upload_field(c, c.w, h.w.data());
+    upload_field(c, c.p[c.pcur], h.p.data());
+    HIPCHK(hipStreamSynchronize(c.stream));
+}
+
+HostFields background(const Ctx& c)
+{
+    // Inamuro::initializeArrays, CPU/Inamuro.cpp:519-524.
+    HostFields h;
+    h.fei.assign(c.n(), c.P.fei_min);
+    h.rho.assign(c.n(), c.P.rho_G);
+    h.u.assign(c.n(), 0.0);
+    h.v.assign(c.n(), 0.0);
+    h.w.assign(c.n(), 0.0);
+    h.p.assign(c.n(), 0.0);
+    return h;
+}
+
+size_t hidx(const Ctx& c, int x, int y, int z)
+{
+    return (static_cast<size_t>(z) * c.P.ly + y) * c.P.lx + x;
+}
+
+} // namespace
+
+extern "C" {
+
+int lbm_reset_fields(void* ctx)
+{
+    return guarded(ctx, [&](Ctx& c) { upload_host(c, background(c)); });
+}
+
+int lbm_init_reference(void* ctx)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        // Inamuro::initializeDroplets, CPU/Inamuro.cpp:550-587.
+        HostFields h = background(c);
+        const double radius = c.P.DD / 2.0;
+        const double cy = c.P.ly / 2.0;
+        const double cz1 = c.P.lz / 2.0 + c.P.DD;
+        const double cz2 = c.P.lz / 2.0 - c.P.DD;
+        const double velocity = 0.035;
+        for (int x = 0; x < c.P.lx; ++x)
+            for (int y = 0; y < c.P.ly; ++y)
+                for (int z = 0; z < c.P.lz; ++z) {
+                    const double dx = x, dy = y - cy;
+                    const double dz1 = z - cz1, dz2 = z - cz2;
+                    const double r1 = std::sqrt(dx * dx + dy * dy + dz1 * dz1);
+                    const double r2 = std::sqrt(dx * dx + dy * dy + dz2 * dz2);
+                    const size_t i = hidx(c, x, y, z);
+                    if (r1 < radius) {
+                        h.fei[i] = c.P.fei_max;
+                        h.rho[i] = c.P.rho_L;
+                        h.w[i] = -velocity;
+                    }
+                    if (r2 < radius) {
+                        h.fei[i] = c.P.fei_max;
+                        h.rho[i] = c.P.rho_L;
+                        h.w[i] = velocity;
+                    }
+                }
+        upload_host(c, h);
+    });
+}
+
+int lbm_place_droplet(void* ctx, double cx, double cy, double cz, double vx, double vy, double vz,
+                      int profile, double interface_width)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        // place_droplet, GPU/Inamuro.cpp:499-533.
+        HostFields h = download_host(c);
+        const double radius = c.P.DD / 2.0;
+        const double width = interface_width > 0.0 ? interface_width : 2.0;
+        for (int x = 0; x < c.P.lx; ++x)
+            for (int y = 0; y < c.P.ly; ++y)
+                for (int z = 0; z < c.P.lz; ++z) {
+                    const double dx = x - cx;
+                    const double dy = y - cy;
+                    const double dz = z - cz;
+                    const double r = std::sqrt(dx * dx + dy * dy + dz * dz);
+                    double lf = 0.0;
+                    if (profile == 1)
+                        lf = 0.5 * (1.0 - std::tanh((r - radius) / width));
+                    else if (r < radius)
+                        lf = 1.0;
+                    if (lf > 0.0) {
+                        const double local_fei = c.P.fei_min + lf * (c.P.fei_max - c.P.fei_min);
+                        const size_t i = hidx(c, x, y, z);
+                        if (local_fei > h.fei[i]) {
+                            h.fei[i] = local_fei;
+                            h.rho[i] = c.P.rho_G + lf * (c.P.rho_L - c.P.rho_G);
+                            if (vx != 0.0) h.u[i] = vx * lf;
+                            if (vy != 0.0) h.v[i] = vy * lf;
+                            h.w[i] = vz * lf;
+                        }
+                    }
+                }
+        upload_host(c, h);
+    });
+}
+
+
+int lbm_create(const lbm_params_t* params, int precision, void** out_ctx)
+{
+    if (!params || !out_ctx) {
+        g_create_error = "null argument";
+        return LBM_ERR_ARG;
+    }
+    *out_ctx = nullptr;
+    try {
+        *out_ctx = create(*params, precision);
+        g_create_error.clear();
+        return LBM_OK;
+    } catch (const ApiError& e) {
+        g_create_error = e.msg;
+        return e.code;
+    } catch (const HipError& e) {
+        g_create_error = std::string("HIP failure: ") + hipGetErrorString(e.code) + " in " + e.what;
+        return LBM_ERR_HIP;
+    } catch (const std::bad_alloc&) {
+        g_create_error = "host allocation failed";
+        return LBM_ERR_NOMEM;
+    }
+}
+
+int lbm_destroy(void* ctx)
+{
+    if (!ctx) return LBM_ERR_ARG;
+    Ctx* c = static_cast<Ctx*>(ctx);
+    free_all(*c);
+    delete c;
+    return LBM_OK;
+}
+
+const char* lbm_last_error(const void* ctx)
+{
+    if (!ctx) return g_create_error.c_str();
+    return static_cast<const Ctx*>(ctx)->err.c_str();
+}
+
+int lbm_upload(void* ctx, const double* fei, const double* rho, const double* u, const double* v,
+               const double* w, const double* p)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        reset_state(c);
+        upload_field(c, c.fei, fei);
+        upload_field(c, c.rho, rho);
+        upload_field(c, c.u, u);
+        upload_field(c, c.v, v);
+        upload_field(c, c.w, w);
+        upload_field(c, c.p[c.pcur], p);
+        HIPCHK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int lbm_upload_dists(void* ctx, const double* f, const double* g, const double* h, int first_step)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        const size_t bytes = lbm::Q * c.n() * sizeof(double);
+        if (f) HIPCHK(hipMemcpyAsync(c.f[c.fcur], f, bytes, hipMemcpyHostToDevice, c.stream));
+        if (g) HIPCHK(hipMemcpyAsync(c.gd[c.fcur], g, bytes, hipMemcpyHostToDevice, c.stream));
+        if (h) {
+            HIPCHK(hipMemcpyAsync(c.h[c.hcur], h, bytes, hipMemcpyHostToDevice, c.stream));
+            c.hs = HState::Explicit;
+        }
+        if (f || g) c.fg = first_step ? FG::First : FG::Local;
+        HIPCHK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance, uint64_t iteration_limit,
+                    int dual_residual)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        // setPoissonConvergence / setPoissonIterationLimit, GPU/InamuroCUDA.cu:2582-2608.
+        if (check_interval <= 0) throw ApiError{LBM_ERR_ARG, "poisson check interval must be positive"};
+        if (!(tolerance > 0.0)) throw ApiError{LBM_ERR_ARG, "poisson tolerance must be positive"};
+        if (iteration_limit != 0 && iteration_limit % static_cast<uint64_t>(check_interval) != 0)
+            throw ApiError{LBM_ERR_ARG, "Poisson iteration limit must be a multiple of check interval"};
+        c.check_interval = check_interval;
+        c.tol = tolerance;
+        c.iter_limit = iteration_limit;
+        c.dual = dual_residual ? 1 : 0;
+    });
+}
+
+int lbm_step(void* ctx, int32_t n)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        if (n < 0) throw ApiError{LBM_ERR_ARG, "step count must be non-negative"};
+        for (int i = 0; i < n; ++i) {
+            if (c.perf.steps == std::numeric_limits<int64_t>::max())
+                throw ApiError{LBM_ERR_OVERFLOW, "time-step counter overflow"};
+            step_one(c);
+        }
+    });
+}
+
+int lbm_download(void* ctx, double* fei, double* rho, double* u, double* v, double* w, double* p)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        download_field(c, fei, c.fei);
+        download_field(c, rho, c.rho);
+        download_field(c, u, c.u);
+        download_field(c, v, c.v);
+        download_field(c, w, c.w);
+        download_field(c, p, c.p[c.pcur]);
+        HIPCHK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int lbm_download_dists(void* ctx, double* f, double* g, double* h)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        const size_t bytes = lbm::Q * c.n() * sizeof(double);
+        if (!c.scratch) c.scratch = dalloc(lbm::Q * c.n());
+        auto get_fg = [&](double* out, double* const* bufs) {
+            if (!out) return;
+            if (c.fg == FG::Post) {
+                lbm::f64::materialize(c.stream, c.g, bufs[c.fcur], c.scratch);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(out, c.scratch, bytes, hipMemcpyDeviceToHost, c.stream));
+            } else {
+                HIPCHK(hipMemcpyAsync(out, bufs[c.fcur], bytes, hipMemcpyDeviceToHost, c.stream));
+            }
+            HIPCHK(hipStreamSynchronize(c.stream));
+        };
+        get_fg(f, c.f);
+        get_fg(g, c.gd);
+        if (h) {
+            if (c.hs == HState::Equilibrium) {
+                lbm::f64::equilibrium_h(c.stream, c.g, c.p[c.pcur], c.scratch);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(h, c.scratch, bytes, hipMemcpyDeviceToHost, c.stream));
+            } else {
+                HIPCHK(hipMemcpyAsync(h, c.h[c.hcur], bytes, hipMemcpyDeviceToHost, c.stream));
+            }
+            HIPCHK(hipStreamSynchronize(c.stream));
+        }
+    });
+}
+
+int lbm_poisson_diag(const void* ctx, lbm_poisson_diag_t* out)
+{
+    if (!ctx || !out) return LBM_ERR_ARG;
+    *out = static_cast<const Ctx*>(ctx)->diag;
+    return LBM_OK;
+}
+
+int lbm_synchronize(void* ctx)
+{
+    return guarded(ctx, [&](Ctx& c) { HIPCHK(hipStreamSynchronize(c.stream)); });
+}
+
+int lbm_set_timing(void* ctx, int enabled)
+{
+    return guarded(ctx, [&](Ctx& c) { c.timing = enabled != 0; });
+}
+
+int lbm_perf_get(const void* ctx, lbm_perf_t* out)
+{
+    if (!ctx || !out) return LBM_ERR_ARG;
+    *out = static_cast<const Ctx*>(ctx)->perf;
+    return LBM_OK;
+}
+
+int lbm_perf_reset(void* ctx)
+{
+    return guarded(ctx, [&](Ctx& c) { c.perf = lbm_perf_t{}; });
+}
+
+int lbm_mass(void* ctx, double* sum_phi)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        if (!sum_phi) throw ApiError{LBM_ERR_ARG, "null argument"};
+        lbm::f64::sum_field(c.stream, c.g, c.fei, c.partials, c.dst);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(c.hst, c.dst, sizeof(DevState), hipMemcpyDeviceToHost, c.stream));
+        HIPCHK(hipStreamSynchronize(c.stream));
+        *sum_phi = c.hst->mass;
+    });
+}
+
+} // extern "C"

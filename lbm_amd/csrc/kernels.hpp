@@ -1,0 +1,96 @@
+// kernels.hpp -- launchers shared by the host orchestration (capi.cpp) and the
+// kernel translation units (kernels_f64.hip, kernels_mixed.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "d3q15.hpp"
+
+namespace lbm {
+
+// Physical constants of one simulation, passed by value to every kernel.
+struct Phys {
+    double rho_L, rho_G, mu_L, mu_G, tauf, taug, k_f, k_g, T, a, b;
+    double fei_L, fei_G;
+    double tmp1;     // (mu_L - mu_G) / (rho_L - rho_G), CPU/Inamuro.cpp:137
+    double fei_avg;  // (fei_L + fei_G) / 2.0, CPU/Inamuro.cpp:296
+    double cell_count;
+};
+
+// Device-resident state of one Poisson solve; the host reads it back once per
+// convergence check.  Status codes name the reference's error messages.
+enum : int32_t {
+    ST_OK = 0,
+    ST_SOURCE_NONFINITE = 1,   // "Poisson source/gauge contains NaN/Inf"
+    ST_GAUGE_NONFINITE = 2,    // "pressure gauge is NaN/Inf"
+    ST_PRESSURE_NONFINITE = 3, // "Poisson pressure/residual contains NaN/Inf"
+    ST_RESIDUAL_NONFINITE = 4, // "Poisson residual is NaN/Inf"
+    ST_FP_NONFINITE = 5,       // "kinetic fixed-point residual contains NaN/Inf"
+    ST_FP_REL_NONFINITE = 6,   // "normalized kinetic fixed-point residual contains NaN/Inf"
+};
+
+struct DevState {
+    double div_mean;
+    double gauge_target;
+    double shift;          // gauge shift of the latest check
+    double max_shift;
+    double e1, e2, rel;    // pressure relative change of the latest check
+    double fp[6];          // dual-residual sums
+    double fp_h_rel, fp_p_rel, fp_rel;
+    double mass;           // scratch for lbm_mass
+    int32_t status;
+    int32_t converged;
+    int32_t pressure_converged;
+    int32_t fp_converged;
+    int32_t fp_evaluated;
+    int32_t pad;
+};
+
+// How a Poisson iteration obtains h and p at its own cell.
+//   H_STREAMED:    h from the streamed buffer with the x-slip rule applied on
+//                  read; p = sum_q h_q (the previous iteration's getp)
+//   H_EQUILIBRIUM: h = p * E_q (update_hh of the previous step); p from buffer
+//   H_GAUGED:      streamed h with the slip rule, minus E_q * shift; gauged p
+//                  from buffer (first iteration after a check)
+//   H_EXPLICIT:    h from buffer as uploaded (already post-boundary); p from
+//                  buffer (first iteration after lbm_upload)
+enum : int { H_STREAMED = 0, H_EQUILIBRIUM = 1, H_GAUGED = 2, H_EXPLICIT = 3 };
+// Collision modes for f and g.
+enum : int { FG_FIRST = 0, FG_PULL = 1, FG_LOCAL = 2 };
+
+inline int grid_blocks(int64_t n) { return static_cast<int>((n + BLOCK - 1) / BLOCK); }
+inline int red_blocks(int64_t n)
+{
+    const int64_t b = (n + BLOCK - 1) / BLOCK;
+    return static_cast<int>(b < RED_BLOCKS ? b : RED_BLOCKS);
+}
+
+namespace f64 {
+
+void collide(hipStream_t s, int mode, const Grid& g, const Phys& ph,
+             const double* fei, const double* rho, const double* u, const double* v,
+             const double* w, const double* p,
+             const double* fin, const double* gin, double* fout, double* gout,
+             double* ux, double* vy, double* wz, double* partials);
+void source_finalize(hipStream_t s, const Grid& g, const Phys& ph, const double* partials,
+                     int nparts, DevState* st);
+void macro(hipStream_t s, int pull, const Grid& g, const Phys& ph, const double* fin,
+           const double* gin, const double* ux, const double* vy, const double* wz,
+           const DevState* st, double* fei, double* rho, double* u, double* v, double* w,
+           double* tau, double* src);
+void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, double* hout,
+                  const double* p, const double* tau, const double* src, const DevState* st);
+void check(hipStream_t s, const Grid& g, const Phys& ph, double* p, double* p_prev,
+           const double* hin, const double* tau, const double* src, int dual, double tol,
+           double* partials, DevState* st);
+void streamed_to_post_boundary(hipStream_t s, const Grid& g, const double* hin, double* out);
+void correct(hipStream_t s, const Grid& g, const double* p, const double* rho, double* u,
+             double* v, double* w);
+void materialize(hipStream_t s, const Grid& g, const double* in, double* out);
+void equilibrium_h(hipStream_t s, const Grid& g, const double* p, double* h);
+void sum_field(hipStream_t s, const Grid& g, const double* a, double* partials, DevState* st);
+
+} // namespace f64
+
+} // namespace lbm

@@ -1,0 +1,312 @@
+"""ctypes binding of lbm_amd/liblbm_hip.so (the C ABI in include/lbm_hip.h).
+
+`InamuroHIP` mirrors the reference's GPU backend class InamuroCUDA
+(GPU/InamuroCUDA.hpp:16-91): performTimeStepGPU, downloadFieldsToCPU,
+downloadHH, getLastPoissonDiagnostics, setPoissonConvergence,
+setPoissonIterationLimit, setUsePoissonDualResidual, synchronize.  Errors
+raise the exception type the reference throws, with the reference's message.
+
+Arrays cross the boundary in the canonical layout: macro fields
+float64[lz, ly, lx], distributions float64[15, lz, ly, lx].
+
+There is no CPU fallback: if the HIP library is missing or no GPU is
+present, construction fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblbm_hip.so")
+
+Q = 15
+FP64 = 0
+MIXED = 1
+
+LBM_OK = 0
+LBM_ERR_ARG = -1
+LBM_ERR_HIP = -2
+LBM_ERR_NUMERIC = -3
+LBM_ERR_OVERFLOW = -4
+LBM_ERR_STATE = -5
+LBM_ERR_NOMEM = -6
+
+# Every symbol include/lbm_hip.h declares.
+EXPORTED = (
+    "lbm_create", "lbm_destroy", "lbm_last_error", "lbm_upload", "lbm_upload_dists",
+    "lbm_set_poisson", "lbm_step", "lbm_download", "lbm_download_dists",
+    "lbm_poisson_diag", "lbm_synchronize", "lbm_set_timing", "lbm_perf_get",
+    "lbm_perf_reset", "lbm_mass", "lbm_init_reference", "lbm_place_droplet",
+    "lbm_reset_fields",
+)
+
+
+class LbmParams(ctypes.Structure):
+    """lbm_params_t: the 14 positional parameters of params.in."""
+    _fields_ = [("lx", ctypes.c_int32), ("ly", ctypes.c_int32), ("lz", ctypes.c_int32)] + [
+        (n, ctypes.c_double) for n in (
+            "rho_L", "rho_G", "mu_L", "mu_G", "tauf", "taug", "k_f", "k_g", "T", "a", "b",
+            "fei_max", "fei_min", "fei_L", "fei_G", "DD")
+    ]
+
+
+class PoissonStepDiagnostics(ctypes.Structure):
+    """lbm_poisson_diag_t (GPU/InamuroCUDA.hpp:19-43)."""
+    _fields_ = [
+        ("iterations", ctypes.c_uint64), ("prev_iterations", ctypes.c_uint64),
+        ("pressure_solve_call_count", ctypes.c_int32), ("hit_max_iterations", ctypes.c_int32),
+        ("relative_error", ctypes.c_double), ("pressure_l1_delta", ctypes.c_double),
+        ("pressure_l1_norm", ctypes.c_double), ("fixed_point_h_l1", ctypes.c_double),
+        ("fixed_point_h_scale", ctypes.c_double), ("fixed_point_h_relative", ctypes.c_double),
+        ("fixed_point_p_l1", ctypes.c_double), ("fixed_point_p_scale", ctypes.c_double),
+        ("fixed_point_p_relative", ctypes.c_double), ("fixed_point_relative", ctypes.c_double),
+        ("source_mean", ctypes.c_double), ("divergence_mean", ctypes.c_double),
+        ("pressure_gauge_target", ctypes.c_double), ("pressure_gauge_max_shift", ctypes.c_double),
+        ("fixed_point_evaluated", ctypes.c_int32), ("pressure_converged", ctypes.c_int32),
+        ("fixed_point_converged", ctypes.c_int32), ("dual_residual_enabled", ctypes.c_int32),
+        ("converged", ctypes.c_int32), ("reserved", ctypes.c_int32),
+    ]
+
+
+class PerformanceMetrics(ctypes.Structure):
+    """lbm_perf_t (GPU/InamuroCUDA.hpp:115-132)."""
+    _fields_ = [
+        ("steps", ctypes.c_int64), ("poisson_iterations", ctypes.c_uint64),
+        ("poisson_checks", ctypes.c_uint64), ("collide_ms", ctypes.c_double),
+        ("macro_ms", ctypes.c_double), ("poisson_ms", ctypes.c_double),
+        ("check_ms", ctypes.c_double), ("correct_ms", ctypes.c_double),
+        ("poisson_iter_launches", ctypes.c_uint64),
+    ]
+
+
+_lib = None
+
+
+def load_library():
+    """Loads liblbm_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"HIP library not built: {LIB_PATH} (run python -c 'import __graft_entry__ as g; g.build()')")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, dp = ctypes.c_void_p, ctypes.POINTER(ctypes.c_double)
+    L.lbm_create.argtypes = [ctypes.POINTER(LbmParams), ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+    L.lbm_destroy.argtypes = [vp]
+    L.lbm_last_error.restype = ctypes.c_char_p
+    L.lbm_last_error.argtypes = [vp]
+    L.lbm_upload.argtypes = [vp] + [dp] * 6
+    L.lbm_upload_dists.argtypes = [vp, dp, dp, dp, ctypes.c_int]
+    L.lbm_set_poisson.argtypes = [vp, ctypes.c_int32, ctypes.c_double, ctypes.c_uint64, ctypes.c_int]
+    L.lbm_step.argtypes = [vp, ctypes.c_int32]
+    L.lbm_download.argtypes = [vp] + [dp] * 6
+    L.lbm_download_dists.argtypes = [vp, dp, dp, dp]
+    L.lbm_poisson_diag.argtypes = [vp, ctypes.POINTER(PoissonStepDiagnostics)]
+    L.lbm_synchronize.argtypes = [vp]
+    L.lbm_set_timing.argtypes = [vp, ctypes.c_int]
+    L.lbm_perf_get.argtypes = [vp, ctypes.POINTER(PerformanceMetrics)]
+    L.lbm_perf_reset.argtypes = [vp]
+    L.lbm_mass.argtypes = [vp, dp]
+    L.lbm_init_reference.argtypes = [vp]
+    L.lbm_reset_fields.argtypes = [vp]
+    L.lbm_place_droplet.argtypes = [vp] + [ctypes.c_double] * 6 + [ctypes.c_int, ctypes.c_double]
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+@dataclass
+class SolverParams:
+    """The physical parameters of params.in (CPU/Inamuro.hpp:39-68 defaults)."""
+    lx: int = 48
+    ly: int = 96
+    lz: int = 128
+    rho_L: float = 50.0
+    rho_G: float = 1.0
+    mu_L: float = 0.008
+    mu_G: float = 0.00016
+    tauf: float = 1.0
+    taug: float = 1.0
+    k_f: float = 0.5
+    k_g: float = 0.0005
+    T: float = 0.035
+    a: float = 1.0
+    b: float = 6.7
+    fei_max: float = 0.09714
+    fei_min: float = 0.01134
+    fei_L: float = 0.092
+    fei_G: float = 0.015
+    DD: float = 32.0
+
+    def to_c(self) -> LbmParams:
+        c = LbmParams()
+        for name, _ in LbmParams._fields_:
+            setattr(c, name, getattr(self, name))
+        return c
+
+
+def _raise(code: int, msg: str):
+    if code == LBM_ERR_ARG:
+        raise ValueError(msg)
+    if code == LBM_ERR_OVERFLOW:
+        raise OverflowError(msg)
+    raise RuntimeError(msg)
+
+
+class InamuroHIP:
+    """GPU time step of the Inamuro two-phase D3Q15 solver (one context)."""
+
+    def __init__(self, params, precision: int = FP64):
+        L = load_library()
+        self.params = params
+        self.precision = precision
+        self._cp = params.to_c() if hasattr(params, "to_c") else SolverParams(
+            **{k: getattr(params, k) for k, _ in LbmParams._fields_}).to_c()
+        h = ctypes.c_void_p()
+        rc = L.lbm_create(ctypes.byref(self._cp), precision, ctypes.byref(h))
+        if rc != LBM_OK:
+            _raise(rc, L.lbm_last_error(None).decode())
+        self._h = h
+        self.shape = (self._cp.lz, self._cp.ly, self._cp.lx)
+
+    # -- lifetime -----------------------------------------------------------
+    def close(self):
+        if getattr(self, "_h", None):
+            load_library().lbm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, rc):
+        if rc != LBM_OK:
+            _raise(rc, load_library().lbm_last_error(self._h).decode())
+
+    # -- data movement ------------------------------------------------------
+    def _field(self, a):
+        if a is None:
+            return None
+        return np.ascontiguousarray(np.broadcast_to(np.asarray(a, dtype=np.float64), self.shape))
+
+    def initFromCPU(self, fei=None, rho=None, u=None, v=None, w=None, p=None):
+        arrs = [self._field(x) for x in (fei, rho, u, v, w, p)]
+        self._check(load_library().lbm_upload(self._h, *[_ptr(a) for a in arrs]))
+
+    def upload_fields(self, fields: dict):
+        self.initFromCPU(**{k: fields.get(k) for k in ("fei", "rho", "u", "v", "w", "p")})
+
+    def upload_dists(self, f=None, g=None, h=None, first_step=False):
+        shp = (Q,) + self.shape
+        arrs = [None if a is None else np.ascontiguousarray(np.asarray(a, dtype=np.float64).reshape(shp))
+                for a in (f, g, h)]
+        self._check(load_library().lbm_upload_dists(self._h, *[_ptr(a) for a in arrs], int(first_step)))
+
+    def downloadFieldsToCPU(self) -> dict:
+        out = {k: np.empty(self.shape) for k in ("fei", "rho", "u", "v", "w", "p")}
+        self._check(load_library().lbm_download(self._h, *[_ptr(out[k]) for k in ("fei", "rho", "u", "v", "w", "p")]))
+        return out
+
+    def download_dists(self, which=("f", "g", "h")) -> dict:
+        shp = (Q,) + self.shape
+        out = {k: np.empty(shp) for k in which}
+        self._check(load_library().lbm_download_dists(
+            self._h, _ptr(out.get("f")), _ptr(out.get("g")), _ptr(out.get("h"))))
+        return out
+
+    def downloadHH(self) -> np.ndarray:
+        return self.download_dists(("h",))["h"]
+
+    def initializeDroplets(self):
+        """CPU-tree initialisation: two sharp droplets (CPU/Inamuro.cpp:550-587)."""
+        self._check(load_library().lbm_init_reference(self._h))
+
+    def reset_fields(self):
+        """Background state of Inamuro::initializeArrays (CPU/Inamuro.cpp:511-548)."""
+        self._check(load_library().lbm_reset_fields(self._h))
+
+    def place_droplet(self, cx, cy, cz, vx=0.0, vy=0.0, vz=0.0, profile="tanh", width=2.0):
+        """GPU-tree place_droplet (GPU/Inamuro.cpp:499-533)."""
+        prof = {"sharp": 0, "tanh": 1}[profile]
+        self._check(load_library().lbm_place_droplet(self._h, cx, cy, cz, vx, vy, vz, prof, width))
+
+    # -- configuration ------------------------------------------------------
+    def setPoisson(self, check_interval=100, tolerance=1e-3, iteration_limit=0, dual_residual=False):
+        self._check(load_library().lbm_set_poisson(self._h, int(check_interval), float(tolerance),
+                                                   int(iteration_limit), int(bool(dual_residual))))
+        self._poisson = (int(check_interval), float(tolerance), int(iteration_limit), bool(dual_residual))
+
+    def _cfg(self):
+        return getattr(self, "_poisson", (100, 1e-3, 0, False))
+
+    def setPoissonConvergence(self, check_interval: int, tolerance: float):
+        _, _, lim, dual = self._cfg()
+        self.setPoisson(check_interval, tolerance, lim, dual)
+
+    def setPoissonIterationLimit(self, max_iterations: int):
+        ci, tol, _, dual = self._cfg()
+        self.setPoisson(ci, tol, max_iterations, dual)
+
+    def setUsePoissonDualResidual(self, enabled: bool):
+        ci, tol, lim, _ = self._cfg()
+        self.setPoisson(ci, tol, lim, enabled)
+
+    # -- stepping -----------------------------------------------------------
+    def performTimeStepGPU(self, n: int = 1):
+        self._check(load_library().lbm_step(self._h, int(n)))
+
+    def step(self, n: int = 1) -> list:
+        """Runs n steps; returns the Poisson iteration count of each."""
+        its = []
+        for _ in range(n):
+            self.performTimeStepGPU(1)
+            its.append(self.getLastPoissonDiagnostics().iterations)
+        return its
+
+    def synchronize(self):
+        self._check(load_library().lbm_synchronize(self._h))
+
+    def getLastPoissonDiagnostics(self) -> PoissonStepDiagnostics:
+        d = PoissonStepDiagnostics()
+        self._check(load_library().lbm_poisson_diag(self._h, ctypes.byref(d)))
+        return d
+
+    def getGridSize(self):
+        return (self._cp.lx, self._cp.ly, self._cp.lz)
+
+    def getAlgorithmName(self) -> str:
+        return "Inamuro"
+
+    # -- timing ---------------------------------------------------------------
+    def set_timing(self, enabled: bool):
+        self._check(load_library().lbm_set_timing(self._h, int(bool(enabled))))
+
+    def perf(self) -> PerformanceMetrics:
+        p = PerformanceMetrics()
+        self._check(load_library().lbm_perf_get(self._h, ctypes.byref(p)))
+        return p
+
+    def reset_perf(self):
+        self._check(load_library().lbm_perf_reset(self._h))
+
+    def mass(self) -> float:
+        out = ctypes.c_double()
+        self._check(load_library().lbm_mass(self._h, ctypes.byref(out)))
+        return out.value
