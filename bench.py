@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Throughput of the Inamuro two-phase D3Q15 time step on one MI355X.
+
+Workload (default): the 256^3 static droplet, R = 64, density ratio 1000,
+tanh profile of width 4 (interface_width 2), phi_L = 0.093, phi_G = 0.016,
+a = 1, b = 6.7, T = 0.035, kappa_f = 0.5, kappa_g = 1e-4, tau_f = tau_g = 1,
+mu_L = 0.008, mu_G = 8e-5, x-slip walls as in the reference, Poisson
+check interval 100 and tolerance 1e-3 (the reference contract).  A step is
+one full performTimeStep: collision + streaming + macro + the Poisson solve
+iterated to convergence + velocity correction.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--precision fp64]
+
+Prints one JSON line.  `value` is MLUPS = cells * steps / wall seconds with
+every input resident in HBM.  `roofline` is the dominant kernel (the Poisson
+iteration): algorithmic bytes per launch (cells x bytes-per-cell of the
+Poisson iteration byte model) over its average launch time measured with
+HIP events on the library's stream.  `cpu_baseline` times the CPU oracle
+(single thread) on a bounded sample of the same 256^3 state.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "MLUPS & % HBM roofline at 256³, FP64 vs mixed precision; adjoint/forward cost"
+HBM_PEAK_GBS = 8000.0
+# Algorithmic bytes per cell of one Poisson iteration (h read+write, p
+# read+write, rho and source read): FP64 15*8*2 + 8*2 + 8*2, mixed
+# 15*4*2 + 8*2 + 4*2.
+POISSON_BYTES = {"fp64": 272, "mixed": 144}
+# Bytes per cell per step outside the Poisson iterations, and per check.
+STEP_BYTES = {"fp64": 632, "mixed": 320}
+CHECK_BYTES = 24
+
+PHYSICS = dict(rho_L=1000.0, rho_G=1.0, mu_L=0.008, mu_G=8e-5, tauf=1.0, taug=1.0, k_f=0.5,
+               k_g=1e-4, T=0.035, a=1.0, b=6.7, fei_max=0.093, fei_min=0.016, fei_L=0.093,
+               fei_G=0.016)
+
+WORKLOADS = {
+    "static_droplet_256": dict(L=256, R=64),
+    "static_droplet_64": dict(L=64, R=16),
+    "static_droplet_128": dict(L=128, R=32),
+}
+
+
+def make_sim(hip, L, R, precision):
+    P = hip.SolverParams(lx=L, ly=L, lz=L, DD=2.0 * R, **PHYSICS)
+    sim = hip.InamuroHIP(P, precision={"fp64": hip.FP64, "mixed": hip.MIXED}[precision])
+    sim.reset_fields()
+    sim.place_droplet(L / 2.0, L / 2.0, L / 2.0, profile="tanh", width=2.0)
+    return sim, P
+
+
+def run_precision(hip, L, R, precision, steps, warmup):
+    sim, P = make_sim(hip, L, R, precision)
+    sim.set_timing(True)
+    for _ in range(warmup):
+        sim.performTimeStepGPU(1)
+    sim.synchronize()
+    sim.reset_perf()
+    iters = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.performTimeStepGPU(1)
+        iters.append(int(sim.getLastPoissonDiagnostics().iterations))
+    sim.synchronize()
+    t1 = time.perf_counter()
+    pf = sim.perf()
+    cells = L * L * L
+    wall = t1 - t0
+    avg_iter_ms = pf.poisson_ms / max(pf.poisson_iter_launches, 1)
+    bpi = POISSON_BYTES[precision] * cells
+    achieved = bpi / (avg_iter_ms * 1e-3) / 1e9
+    n_it = sum(iters)
+    n_chk = n_it // 100
+    step_bytes = cells * (STEP_BYTES[precision] * steps + POISSON_BYTES[precision] * n_it + CHECK_BYTES * n_chk)
+    out = dict(
+        precision=precision, wall_s=wall, mlups=cells * steps / wall / 1e6,
+        ms_per_step=wall / steps * 1e3, iterations=iters,
+        poisson_iter_ms=avg_iter_ms, poisson_iter_launches=int(pf.poisson_iter_launches),
+        mlups_h=cells * n_it / (pf.poisson_ms * 1e-3) / 1e6 if pf.poisson_ms > 0 else None,
+        achieved_gbs=achieved, bytes_per_launch=bpi,
+        whole_step_gbs=step_bytes / wall / 1e9,
+        phase_ms=dict(collide=pf.collide_ms, macro=pf.macro_ms, poisson=pf.poisson_ms,
+                      check=pf.check_ms, correct=pf.correct_ms),
+    )
+    sim.close()
+    return out
+
+
+def cpu_baseline(L, R, n_it_per_step):
+    """Single-thread CPU oracle on the same 256^3 state: one non-Poisson step
+    part plus two Poisson iterations, extrapolated with the GPU's measured
+    iterations per step."""
+    from oracle import pyoracle as po
+
+    P = po.Params(lx=L, ly=L, lz=L, DD=2.0 * R, **PHYSICS)
+    o = po.Oracle(P, threads=1)
+    o.place_droplet(L / 2.0, L / 2.0, L / 2.0, profile="tanh", width=2.0)
+    t0 = time.perf_counter()
+    for st in ("collision", "stream_fg", "bc_fg", "macro"):
+        o.stage(st)
+    t1 = time.perf_counter()
+    k = 2
+    o.poisson_iterations(k)
+    t2 = time.perf_counter()
+    o.stage("correct")
+    o.stage("update_hh")
+    t3 = time.perf_counter()
+    t_np = (t1 - t0) + (t3 - t2)
+    t_it = (t2 - t1) / k
+    t_step = t_np + n_it_per_step * t_it
+    o.close()
+    return dict(
+        value=L ** 3 / t_step / 1e6, unit="MLUPS", cores=1, kind="port",
+        sample=(f"{L}^3 static-droplet state: one collision+stream+boundary+macro+correction pass "
+                f"({t_np:.2f} s) and {k} Poisson iterations ({t_it:.3f} s each) timed on one host core, "
+                f"extrapolated to {n_it_per_step:.0f} iterations per step (the GPU's measured mean)"),
+        host_cpu=platform.processor() or platform.machine(), nproc=os.cpu_count(),
+    )
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="fp64")
+    ap.add_argument("--workload", choices=sorted(WORKLOADS), default="static_droplet_256")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: the time step is not sharded across GPUs in this "
+              f"build (asked for --gpus {args.gpus})", file=sys.stderr)
+        return 2
+
+    from lbm_amd import hip
+
+    wl = WORKLOADS[args.workload]
+    L, R = wl["L"], wl["R"]
+    precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    results = {p: run_precision(hip, L, R, p, args.steps, args.warmup) for p in precs}
+    head = results["mixed"] if "mixed" in results else results["fp64"]
+    n_it_mean = sum(head["iterations"]) / len(head["iterations"])
+    line = {
+        "metric": METRIC,
+        "value": head["mlups"],
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if head["precision"] == "fp64" else "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{args.workload}: {L}^3 D3Q15 grid, tanh droplet R={R} at the centre, "
+                        "density ratio 1000, Poisson tol 1e-3 / check 100, steps after a cold start",
+            "grid": [L, L, L], "radius": R, "precision": head["precision"],
+            "poisson_iterations_per_step": head["iterations"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": head["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": head["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_poisson_push (one Poisson LBM iteration)",
+            "algorithmic_bytes_per_launch": head["bytes_per_launch"],
+            "avg_launch_ms": head["poisson_iter_ms"],
+        },
+        "mlups_poisson": head["mlups_h"],
+        "whole_step_hbm_frac": head["whole_step_gbs"] / HBM_PEAK_GBS,
+        "phase_ms": head["phase_ms"],
+    }
+    if "mixed" in results and "fp64" in results:
+        line["fp64_mlups"] = results["fp64"]["mlups"]
+        line["fp64_roofline_frac"] = results["fp64"]["achieved_gbs"] / HBM_PEAK_GBS
+        line["mixed_over_fp64_mlups"] = results["mixed"]["mlups"] / results["fp64"]["mlups"]
+        line["mixed_over_fp64_iteration_time"] = results["fp64"]["poisson_iter_ms"] / results["mixed"]["poisson_iter_ms"]
+    traffic = os.path.join(ROOT, "profiles", f"traffic_{head['precision']}_{args.workload}.json")
+    if os.path.exists(traffic):
+        line["roofline"]["traffic"] = json.load(open(traffic)).get("bytes_per_launch")
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(L, R, n_it_mean)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -27,6 +27,7 @@ COMMON = [
 # (source, extra flags)
 UNITS = [
     ("kernels_f64.hip", ["-ffp-contract=off"]),
+    ("kernels_mixed.hip", ["-ffp-contract=fast"]),
     ("capi.cpp", ["-x", "hip", "-ffp-contract=off"]),
 ]
 

@@ -4,10 +4,10 @@
 // semantics (CPU/Inamuro.cpp:347-363, 624-724):
 //
 //   collide (stencils + BGK + pull of f,g)  -> source/gauge-target reduction
-//   -> macro (pull of f,g, rho map, tau_h, projected source)
-//   -> Poisson: segments of `check_interval` fused iterations, then one
-//      on-device check (gauge, relative change, optional dual residual) and a
-//      single device-to-host read of a small state struct
+//   -> macro (pull of f,g, rho map, tau_h or 1-1/tau_h, projected source)
+//   -> Poisson: segments of `check_interval` push-form iterations, then one
+//      on-device check (getp, gauge, relative change, optional dual residual)
+//      and a single device-to-host read of a small state struct
 //   -> velocity correction.  h = E*p is folded into the next step's first
 //      iteration instead of being written.
 //
@@ -16,6 +16,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -56,25 +57,36 @@ enum class HState { Explicit, Equilibrium };
 struct Ctx {
     lbm_params_t P{};
     int precision = LBM_PRECISION_FP64;
+    bool mixed = false;
+    size_t esz = sizeof(double);  // bytes of one stored f/g/h/macro value
     Grid g{};
     Phys ph{};
+    lbm::mx::PhysF phf{};
     int device = 0;
     hipStream_t stream = nullptr;
 
-    // FP64 device buffers.
-    double* f[2] = {nullptr, nullptr};
-    double* gd[2] = {nullptr, nullptr};
-    double* h[2] = {nullptr, nullptr};
+    // Precision-typed device buffers (double in FP64, float in mixed).
+    void* f[2] = {nullptr, nullptr};
+    void* gd[2] = {nullptr, nullptr};
+    void* h[2] = {nullptr, nullptr};
+    void *fei = nullptr, *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+    void *ux = nullptr, *vy = nullptr, *wz = nullptr;
+    void* tau = nullptr;  // FP64: tau_h;  mixed: 1 - 1/tau_h
+    void* src = nullptr;  // FP64: S;      mixed: S/3
+    // FP64 in both precisions (p[1] unused).
     double* p[2] = {nullptr, nullptr};
-    double *fei = nullptr, *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
-    double *tau = nullptr, *src = nullptr, *ux = nullptr, *vy = nullptr, *wz = nullptr;
+    // Mixed: FP32 decode bases of the streamed h, ping-pong.
+    float* base[2] = {nullptr, nullptr};
     double* p_prev = nullptr;
     double* partials = nullptr;
-    double* scratch = nullptr;  // one distribution array, for downloads
+    double* scratch = nullptr;   // 15*n doubles, lazily, for distribution transfers
+    double* scratch1 = nullptr;  // n doubles, mixed field transfers
     DevState* dst = nullptr;
-    DevState* hst = nullptr;    // pinned mirror
+    DevState* hst = nullptr;     // pinned mirror
 
-    int fcur = 0, hcur = 0, pcur = 0;
+    int fcur = 0, hcur = 0;
+    int pc = 0;  // buffer holding the current pressure
+    int pb = 0;  // mixed: base buffer that decodes the streamed h
     FG fg = FG::First;
     HState hs = HState::Explicit;
 
@@ -96,24 +108,23 @@ struct Ctx {
     size_t n() const { return static_cast<size_t>(g.n); }
 };
 
+template <typename T>
+T* D(void* p) { return static_cast<T*>(p); }
+
 void free_all(Ctx& c)
 {
     (void)hipSetDevice(c.device);
-    auto F = [](double*& ptr) {
+    auto F = [](auto*& ptr) {
         if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
     };
     for (int i = 0; i < 2; ++i) {
-        F(c.f[i]);
-        F(c.gd[i]);
-        F(c.h[i]);
-        F(c.p[i]);
+        F(c.f[i]); F(c.gd[i]); F(c.h[i]); F(c.p[i]); F(c.base[i]);
     }
     F(c.fei); F(c.rho); F(c.u); F(c.v); F(c.w);
     F(c.tau); F(c.src); F(c.ux); F(c.vy); F(c.wz);
-    F(c.p_prev); F(c.partials); F(c.scratch);
-    if (c.dst) (void)hipFree(c.dst);
-    c.dst = nullptr;
+    F(c.p_prev); F(c.partials); F(c.scratch); F(c.scratch1);
+    F(c.dst);
     if (c.hst) (void)hipHostFree(c.hst);
     c.hst = nullptr;
     for (auto& e : c.ev) {
@@ -124,32 +135,65 @@ void free_all(Ctx& c)
     c.stream = nullptr;
 }
 
-double* dalloc(size_t count)
+void* dalloc_bytes(size_t bytes)
 {
     void* ptr = nullptr;
-    const hipError_t e = hipMalloc(&ptr, count * sizeof(double));
+    const hipError_t e = hipMalloc(&ptr, bytes);
     if (e != hipSuccess) throw ApiError{LBM_ERR_NOMEM, "device allocation failed"};
-    return static_cast<double*>(ptr);
+    return ptr;
 }
 
-void fill(Ctx& c, double* dst, double value, size_t count)
+double* dalloc(size_t count) { return static_cast<double*>(dalloc_bytes(count * sizeof(double))); }
+
+void sync(Ctx& c) { HIPCHK(hipStreamSynchronize(c.stream)); }
+
+// Host FP64 field -> device field of the context's precision.
+void put_field(Ctx& c, void* dst, const double* src)
 {
-    std::vector<double> tmp(count, value);
-    HIPCHK(hipMemcpyAsync(dst, tmp.data(), count * sizeof(double), hipMemcpyHostToDevice, c.stream));
-    HIPCHK(hipStreamSynchronize(c.stream));
+    if (!src) return;
+    if (!c.mixed) {
+        HIPCHK(hipMemcpyAsync(dst, src, c.n() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+        return;
+    }
+    HIPCHK(hipMemcpyAsync(c.scratch1, src, c.n() * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    lbm::mx::to_f32(c.stream, c.g.n, c.scratch1, D<float>(dst));
+    HIPCHK(hipGetLastError());
+    sync(c);
+}
+
+// Device field -> host FP64.
+void get_field(Ctx& c, double* dst, void* src)
+{
+    if (!dst) return;
+    if (!c.mixed) {
+        HIPCHK(hipMemcpyAsync(dst, src, c.n() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        return;
+    }
+    lbm::mx::to_f64(c.stream, c.g.n, D<float>(src), c.scratch1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst, c.scratch1, c.n() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    sync(c);
+}
+
+void put_p(Ctx& c, const double* src)
+{
+    if (src) HIPCHK(hipMemcpyAsync(c.p[c.pc], src, c.n() * sizeof(double), hipMemcpyHostToDevice, c.stream));
 }
 
 void reset_state(Ctx& c)
 {
-    // Inamuro::initializeArrays, CPU/Inamuro.cpp:511-548.
-    const size_t n = c.n();
-    HIPCHK(hipMemsetAsync(c.h[0], 0, lbm::Q * n * sizeof(double), c.stream));
-    HIPCHK(hipMemsetAsync(c.h[1], 0, lbm::Q * n * sizeof(double), c.stream));
-    HIPCHK(hipMemsetAsync(c.f[0], 0, lbm::Q * n * sizeof(double), c.stream));
-    HIPCHK(hipMemsetAsync(c.gd[0], 0, lbm::Q * n * sizeof(double), c.stream));
-    HIPCHK(hipMemsetAsync(c.p_prev, 0, n * sizeof(double), c.stream));
+    // The state a freshly constructed reference solver has before its first
+    // step (CPU/Inamuro.cpp:511-548): h = 0, f/g unset, p_prev = 0.
+    const size_t dist_bytes = lbm::Q * c.n() * c.esz;
+    for (int i = 0; i < 2; ++i) {
+        HIPCHK(hipMemsetAsync(c.h[i], 0, dist_bytes, c.stream));
+        HIPCHK(hipMemsetAsync(c.f[i], 0, dist_bytes, c.stream));
+        HIPCHK(hipMemsetAsync(c.gd[i], 0, dist_bytes, c.stream));
+    }
+    HIPCHK(hipMemsetAsync(c.p_prev, 0, c.n() * sizeof(double), c.stream));
     HIPCHK(hipMemsetAsync(c.dst, 0, sizeof(DevState), c.stream));
-    c.fcur = c.hcur = c.pcur = 0;
+    c.fcur = c.hcur = 0;
+    c.pb = 0;
     c.fg = FG::First;
     c.hs = HState::Explicit;
     c.last_iterations = 0;
@@ -173,12 +217,12 @@ Ctx* create(const lbm_params_t& P, int precision)
     check_params(P);
     if (precision != LBM_PRECISION_FP64 && precision != LBM_PRECISION_MIXED)
         throw ApiError{LBM_ERR_ARG, "unknown precision"};
-    if (precision != LBM_PRECISION_FP64)
-        throw ApiError{LBM_ERR_ARG, "precision not available in this build"};
     Ctx* c = new Ctx();
     try {
         c->P = P;
         c->precision = precision;
+        c->mixed = precision == LBM_PRECISION_MIXED;
+        c->esz = c->mixed ? sizeof(float) : sizeof(double);
         HIPCHK(hipGetDevice(&c->device));
         c->g = Grid{P.lx, P.ly, P.lz, static_cast<int64_t>(P.lx) * P.ly * P.lz};
         c->ph = Phys{P.rho_L, P.rho_G, P.mu_L, P.mu_G, P.tauf, P.taug, P.k_f, P.k_g, P.T, P.a, P.b,
@@ -186,31 +230,54 @@ Ctx* create(const lbm_params_t& P, int precision)
                      (P.mu_L - P.mu_G) / (P.rho_L - P.rho_G),
                      (P.fei_L + P.fei_G) / 2.0,
                      static_cast<double>(P.lx) * P.ly * P.lz};
+        c->phf = lbm::mx::PhysF{
+            static_cast<float>(P.rho_L), static_cast<float>(P.rho_G), static_cast<float>(P.mu_G),
+            static_cast<float>(1.0 / P.tauf), static_cast<float>(1.0 / P.taug),
+            static_cast<float>(P.k_f), static_cast<float>(P.k_g), static_cast<float>(P.T),
+            static_cast<float>(P.a), static_cast<float>(P.b), static_cast<float>(P.fei_L),
+            static_cast<float>(P.fei_G), static_cast<float>(c->ph.tmp1),
+            static_cast<float>(c->ph.fei_avg), static_cast<float>(1.5 * (P.taug - 0.5) * 2.0)};
         HIPCHK(hipStreamCreate(&c->stream));
         const size_t n = c->n();
+        const size_t fb = n * c->esz;
         for (int i = 0; i < 2; ++i) {
-            c->f[i] = dalloc(lbm::Q * n);
-            c->gd[i] = dalloc(lbm::Q * n);
-            c->h[i] = dalloc(lbm::Q * n);
-            c->p[i] = dalloc(n);
+            c->f[i] = dalloc_bytes(lbm::Q * fb);
+            c->gd[i] = dalloc_bytes(lbm::Q * fb);
+            c->h[i] = dalloc_bytes(lbm::Q * fb);
         }
-        c->fei = dalloc(n); c->rho = dalloc(n); c->u = dalloc(n); c->v = dalloc(n); c->w = dalloc(n);
-        c->tau = dalloc(n); c->src = dalloc(n); c->ux = dalloc(n); c->vy = dalloc(n); c->wz = dalloc(n);
+        c->p[0] = dalloc(n);
+        if (c->mixed) {
+            c->base[0] = static_cast<float*>(dalloc_bytes(n * sizeof(float)));
+            c->base[1] = static_cast<float*>(dalloc_bytes(n * sizeof(float)));
+        }
+        for (void** a : {&c->fei, &c->rho, &c->u, &c->v, &c->w, &c->tau, &c->src, &c->ux, &c->vy, &c->wz})
+            *a = dalloc_bytes(fb);
         c->p_prev = dalloc(n);
+        if (c->mixed) c->scratch1 = dalloc(n);
         const size_t nparts = std::max<size_t>(2 * static_cast<size_t>(lbm::grid_blocks(c->g.n)),
                                                9 * static_cast<size_t>(lbm::RED_BLOCKS));
         c->partials = dalloc(nparts);
-        HIPCHK(hipMalloc(&c->dst, sizeof(DevState)));
+        c->dst = static_cast<DevState*>(dalloc_bytes(sizeof(DevState)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hst), sizeof(DevState), hipHostMallocDefault));
         for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
-        fill(*c, c->rho, P.rho_G, n);
-        fill(*c, c->fei, P.fei_min, n);
-        HIPCHK(hipMemsetAsync(c->u, 0, n * sizeof(double), c->stream));
-        HIPCHK(hipMemsetAsync(c->v, 0, n * sizeof(double), c->stream));
-        HIPCHK(hipMemsetAsync(c->w, 0, n * sizeof(double), c->stream));
-        HIPCHK(hipMemsetAsync(c->p[0], 0, n * sizeof(double), c->stream));
+        c->pc = c->pb = 0;
+        {
+            // Inamuro::initializeArrays, CPU/Inamuro.cpp:519-524.
+            std::vector<double> tmp(n, P.rho_G);
+            put_field(*c, c->rho, tmp.data());
+            sync(*c);
+            std::fill(tmp.begin(), tmp.end(), P.fei_min);
+            put_field(*c, c->fei, tmp.data());
+            sync(*c);
+            std::fill(tmp.begin(), tmp.end(), 0.0);
+            put_field(*c, c->u, tmp.data());
+            put_field(*c, c->v, tmp.data());
+            put_field(*c, c->w, tmp.data());
+            put_p(*c, tmp.data());
+            sync(*c);
+        }
         reset_state(*c);
-        HIPCHK(hipStreamSynchronize(c->stream));
+        sync(*c);
     } catch (...) {
         free_all(*c);
         delete c;
@@ -244,29 +311,96 @@ const char* status_message(int32_t st)
     }
 }
 
-// One performTimeStep.
-void step_one(Ctx& c)
+int fg_mode(const Ctx& c)
+{
+    return c.fg == FG::First ? lbm::FG_FIRST : (c.fg == FG::Post ? lbm::FG_PULL : lbm::FG_LOCAL);
+}
+
+// Collision, source reduction and macro pass of one step.
+void step_fg(Ctx& c)
 {
     hipStream_t s = c.stream;
     const Grid& g = c.g;
-    // 1. collision fused with the pull of the streamed f and g.
-    record(c, 0);
-    const int mode = c.fg == FG::First ? lbm::FG_FIRST : (c.fg == FG::Post ? lbm::FG_PULL : lbm::FG_LOCAL);
+    const int mode = fg_mode(c);
     const int nxt = 1 - c.fcur;
-    lbm::f64::collide(s, mode, g, c.ph, c.fei, c.rho, c.u, c.v, c.w, c.p[c.pcur],
-                      c.f[c.fcur], c.gd[c.fcur], c.f[nxt], c.gd[nxt], c.ux, c.vy, c.wz, c.partials);
+    record(c, 0);
+    if (!c.mixed) {
+        lbm::f64::collide(s, mode, g, c.ph, D<double>(c.fei), D<double>(c.rho), D<double>(c.u),
+                          D<double>(c.v), D<double>(c.w), c.p[c.pc], D<double>(c.f[c.fcur]),
+                          D<double>(c.gd[c.fcur]), D<double>(c.f[nxt]), D<double>(c.gd[nxt]),
+                          D<double>(c.ux), D<double>(c.vy), D<double>(c.wz), c.partials);
+    } else {
+        lbm::mx::collide(s, mode, g, c.phf, D<float>(c.fei), D<float>(c.rho), D<float>(c.u),
+                         D<float>(c.v), D<float>(c.w), c.p[c.pc], D<float>(c.f[c.fcur]),
+                         D<float>(c.gd[c.fcur]), D<float>(c.f[nxt]), D<float>(c.gd[nxt]),
+                         D<float>(c.ux), D<float>(c.vy), D<float>(c.wz), c.partials);
+    }
     HIPCHK(hipGetLastError());
     c.fcur = nxt;
     c.fg = FG::Post;
     record(c, 1);
-    // 2. source statistics, then macro fields from the pulled populations.
     lbm::f64::source_finalize(s, g, c.ph, c.partials, lbm::grid_blocks(g.n), c.dst);
-    lbm::f64::macro(s, 1, g, c.ph, c.f[c.fcur], c.gd[c.fcur], c.ux, c.vy, c.wz, c.dst,
-                    c.fei, c.rho, c.u, c.v, c.w, c.tau, c.src);
+    if (!c.mixed) {
+        lbm::f64::macro(s, 1, g, c.ph, D<double>(c.f[c.fcur]), D<double>(c.gd[c.fcur]),
+                        D<double>(c.ux), D<double>(c.vy), D<double>(c.wz), c.dst, D<double>(c.fei),
+                        D<double>(c.rho), D<double>(c.u), D<double>(c.v), D<double>(c.w),
+                        D<double>(c.tau), D<double>(c.src));
+    } else {
+        lbm::mx::macro(s, g, c.phf, D<float>(c.f[c.fcur]), D<float>(c.gd[c.fcur]), D<float>(c.ux),
+                       D<float>(c.vy), D<float>(c.wz), c.dst, D<float>(c.fei), D<float>(c.rho),
+                       D<float>(c.u), D<float>(c.v), D<float>(c.w), D<float>(c.tau), D<float>(c.src));
+    }
     HIPCHK(hipGetLastError());
     record(c, 2);
+}
 
-    // 3. Poisson solve.
+// One Poisson iteration; updates the buffer bookkeeping.
+void poisson_iteration(Ctx& c, int mode)
+{
+    const Grid& g = c.g;
+    if (!c.mixed) {
+        lbm::f64::poisson_iter(c.stream, mode, g, D<double>(c.h[c.hcur]), D<double>(c.h[1 - c.hcur]),
+                               c.p[c.pc], D<double>(c.tau), D<double>(c.src), c.dst);
+    } else {
+        // Decodes with base pb, writes this iteration's bases into the other
+        // buffer; the cell pressure of the non-streamed modes comes from p.
+        lbm::mx::poisson_iter(c.stream, mode, g, D<float>(c.h[c.hcur]), D<float>(c.h[1 - c.hcur]),
+                              c.base[c.pb], c.p[c.pc], c.base[1 - c.pb], D<float>(c.tau),
+                              D<float>(c.src), c.dst);
+        c.pb = 1 - c.pb;
+    }
+    c.hcur = 1 - c.hcur;
+}
+
+// getp + gauge + relative change (+ dual residual) after a segment.
+void poisson_check(Ctx& c)
+{
+    hipStream_t s = c.stream;
+    const Grid& g = c.g;
+    if (!c.mixed) {
+        lbm::f64::check(s, g, c.ph, c.p[c.pc], c.p_prev, D<double>(c.h[c.hcur]), D<double>(c.tau),
+                        D<double>(c.src), c.dual, c.tol, c.partials, c.dst);
+        return;
+    }
+    const int nb = lbm::red_blocks(g.n);
+    double* parts_sum = c.partials;
+    double* parts_err = c.partials + lbm::RED_BLOCKS;
+    double* parts_fp = c.partials + 3 * lbm::RED_BLOCKS;
+    lbm::mx::getp_sum(s, g, D<float>(c.h[c.hcur]), c.base[c.pb], c.p[c.pc], parts_sum);
+    lbm::f64::gauge_finalize(s, c.ph, parts_sum, nb, c.dst);
+    lbm::f64::gauge_error(s, g, c.p[c.pc], c.p_prev, c.dst, parts_err);
+    if (c.dual)
+        lbm::mx::fixed_point(s, g, D<float>(c.h[c.hcur]), c.base[c.pb], c.p[c.pc], D<float>(c.tau),
+                             D<float>(c.src), c.dst, parts_fp);
+    lbm::f64::check_finalize(s, parts_err, parts_fp, nb, c.dual, c.tol, c.dst);
+}
+
+// One performTimeStep.
+void step_one(Ctx& c)
+{
+    hipStream_t s = c.stream;
+    step_fg(c);
+
     ++c.solve_calls;
     lbm_poisson_diag_t& d = c.diag;
     const uint64_t prev = c.last_iterations;
@@ -288,19 +422,16 @@ void step_one(Ctx& c)
             if (iter == std::numeric_limits<uint64_t>::max())
                 throw ApiError{LBM_ERR_OVERFLOW, "Poisson iteration counter overflow"};
             ++iter;
-            lbm::f64::poisson_iter(s, hmode, g, c.h[c.hcur], c.h[1 - c.hcur], c.p[c.pcur],
-                                   c.tau, c.src, c.dst);
-            c.hcur = 1 - c.hcur;
+            poisson_iteration(c, hmode);
             hmode = lbm::H_STREAMED;
         }
         HIPCHK(hipGetLastError());
         record(c, 5);
-        lbm::f64::check(s, g, c.ph, c.p[c.pcur], c.p_prev, c.h[c.hcur], c.tau, c.src, c.dual, c.tol,
-                        c.partials, c.dst);
+        poisson_check(c);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(c.hst, c.dst, sizeof(DevState), hipMemcpyDeviceToHost, s));
         record(c, 6);
-        HIPCHK(hipStreamSynchronize(s));
+        sync(c);
         if (c.timing) {
             if (first_segment) {
                 c.perf.collide_ms += elapsed(c.ev[0], c.ev[1]);
@@ -345,9 +476,14 @@ void step_one(Ctx& c)
     c.last_iterations = iter;
     c.perf.poisson_iterations += iter;
 
-    // 4. velocity correction; h = E*p is implied for the next step.
+    // Velocity correction; h = E*p is implied for the next step.
     record(c, 7);
-    lbm::f64::correct(s, g, c.p[c.pcur], c.rho, c.u, c.v, c.w);
+    if (!c.mixed)
+        lbm::f64::correct(s, c.g, c.p[c.pc], D<double>(c.rho), D<double>(c.u), D<double>(c.v),
+                          D<double>(c.w));
+    else
+        lbm::mx::correct(s, c.g, c.p[c.pc], D<float>(c.rho), D<float>(c.u), D<float>(c.v),
+                         D<float>(c.w));
     HIPCHK(hipGetLastError());
     record(c, 8);
     c.hs = HState::Equilibrium;
@@ -383,16 +519,6 @@ int guarded(void* ctx, Fn&& fn)
     }
 }
 
-void upload_field(Ctx& c, double* dst, const double* src)
-{
-    if (src) HIPCHK(hipMemcpyAsync(dst, src, c.n() * sizeof(double), hipMemcpyHostToDevice, c.stream));
-}
-
-void download_field(Ctx& c, double* dst, const double* src)
-{
-    if (dst) HIPCHK(hipMemcpyAsync(dst, src, c.n() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-}
-
 struct HostFields {
     std::vector<double> fei, rho, u, v, w, p;
 };
@@ -401,26 +527,26 @@ HostFields download_host(Ctx& c)
 {
     HostFields h;
     for (auto* a : {&h.fei, &h.rho, &h.u, &h.v, &h.w, &h.p}) a->resize(c.n());
-    download_field(c, h.fei.data(), c.fei);
-    download_field(c, h.rho.data(), c.rho);
-    download_field(c, h.u.data(), c.u);
-    download_field(c, h.v.data(), c.v);
-    download_field(c, h.w.data(), c.w);
-    download_field(c, h.p.data(), c.p[c.pcur]);
-    HIPCHK(hipStreamSynchronize(c.stream));
+    get_field(c, h.fei.data(), c.fei);
+    get_field(c, h.rho.data(), c.rho);
+    get_field(c, h.u.data(), c.u);
+    get_field(c, h.v.data(), c.v);
+    get_field(c, h.w.data(), c.w);
+    HIPCHK(hipMemcpyAsync(h.p.data(), c.p[c.pc], c.n() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    sync(c);
     return h;
 }
 
 void upload_host(Ctx& c, const HostFields& h)
 {
     reset_state(c);
-    upload_field(c, c.fei, h.fei.data());
-    upload_field(c, c.rho, h.rho.data());
-    upload_field(c, c.u, h.u.data());
-    upload_field(c, c.v, h.v.data());
-    upload_field(c, c.w, h.w.data());
-    upload_field(c, c.p[c.pcur], h.p.data());
-    HIPCHK(hipStreamSynchronize(c.stream));
+    put_field(c, c.fei, h.fei.data());
+    put_field(c, c.rho, h.rho.data());
+    put_field(c, c.u, h.u.data());
+    put_field(c, c.v, h.v.data());
+    put_field(c, c.w, h.w.data());
+    put_p(c, h.p.data());
+    sync(c);
 }
 
 HostFields background(const Ctx& c)
@@ -441,9 +567,52 @@ size_t hidx(const Ctx& c, int x, int y, int z)
     return (static_cast<size_t>(z) * c.P.ly + y) * c.P.lx + x;
 }
 
+void ensure_scratch(Ctx& c)
+{
+    if (!c.scratch) c.scratch = dalloc(lbm::Q * c.n());
+}
+
 } // namespace
 
 extern "C" {
+
+int lbm_create(const lbm_params_t* params, int precision, void** out_ctx)
+{
+    if (!params || !out_ctx) {
+        g_create_error = "null argument";
+        return LBM_ERR_ARG;
+    }
+    *out_ctx = nullptr;
+    try {
+        *out_ctx = create(*params, precision);
+        g_create_error.clear();
+        return LBM_OK;
+    } catch (const ApiError& e) {
+        g_create_error = e.msg;
+        return e.code;
+    } catch (const HipError& e) {
+        g_create_error = std::string("HIP failure: ") + hipGetErrorString(e.code) + " in " + e.what;
+        return LBM_ERR_HIP;
+    } catch (const std::bad_alloc&) {
+        g_create_error = "host allocation failed";
+        return LBM_ERR_NOMEM;
+    }
+}
+
+int lbm_destroy(void* ctx)
+{
+    if (!ctx) return LBM_ERR_ARG;
+    Ctx* c = static_cast<Ctx*>(ctx);
+    free_all(*c);
+    delete c;
+    return LBM_OK;
+}
+
+const char* lbm_last_error(const void* ctx)
+{
+    if (!ctx) return g_create_error.c_str();
+    return static_cast<const Ctx*>(ctx)->err.c_str();
+}
 
 int lbm_reset_fields(void* ctx)
 {
@@ -519,72 +688,49 @@ int lbm_place_droplet(void* ctx, double cx, double cy, double cz, double vx, dou
     });
 }
 
-
-int lbm_create(const lbm_params_t* params, int precision, void** out_ctx)
-{
-    if (!params || !out_ctx) {
-        g_create_error = "null argument";
-        return LBM_ERR_ARG;
-    }
-    *out_ctx = nullptr;
-    try {
-        *out_ctx = create(*params, precision);
-        g_create_error.clear();
-        return LBM_OK;
-    } catch (const ApiError& e) {
-        g_create_error = e.msg;
-        return e.code;
-    } catch (const HipError& e) {
-        g_create_error = std::string("HIP failure: ") + hipGetErrorString(e.code) + " in " + e.what;
-        return LBM_ERR_HIP;
-    } catch (const std::bad_alloc&) {
-        g_create_error = "host allocation failed";
-        return LBM_ERR_NOMEM;
-    }
-}
-
-int lbm_destroy(void* ctx)
-{
-    if (!ctx) return LBM_ERR_ARG;
-    Ctx* c = static_cast<Ctx*>(ctx);
-    free_all(*c);
-    delete c;
-    return LBM_OK;
-}
-
-const char* lbm_last_error(const void* ctx)
-{
-    if (!ctx) return g_create_error.c_str();
-    return static_cast<const Ctx*>(ctx)->err.c_str();
-}
-
 int lbm_upload(void* ctx, const double* fei, const double* rho, const double* u, const double* v,
                const double* w, const double* p)
 {
     return guarded(ctx, [&](Ctx& c) {
         reset_state(c);
-        upload_field(c, c.fei, fei);
-        upload_field(c, c.rho, rho);
-        upload_field(c, c.u, u);
-        upload_field(c, c.v, v);
-        upload_field(c, c.w, w);
-        upload_field(c, c.p[c.pcur], p);
-        HIPCHK(hipStreamSynchronize(c.stream));
+        put_field(c, c.fei, fei);
+        put_field(c, c.rho, rho);
+        put_field(c, c.u, u);
+        put_field(c, c.v, v);
+        put_field(c, c.w, w);
+        put_p(c, p);
+        sync(c);
     });
 }
 
 int lbm_upload_dists(void* ctx, const double* f, const double* g, const double* h, int first_step)
 {
     return guarded(ctx, [&](Ctx& c) {
-        const size_t bytes = lbm::Q * c.n() * sizeof(double);
-        if (f) HIPCHK(hipMemcpyAsync(c.f[c.fcur], f, bytes, hipMemcpyHostToDevice, c.stream));
-        if (g) HIPCHK(hipMemcpyAsync(c.gd[c.fcur], g, bytes, hipMemcpyHostToDevice, c.stream));
-        if (h) {
-            HIPCHK(hipMemcpyAsync(c.h[c.hcur], h, bytes, hipMemcpyHostToDevice, c.stream));
-            c.hs = HState::Explicit;
+        const size_t n = c.n();
+        const size_t bytes = lbm::Q * n * sizeof(double);
+        if (!c.mixed) {
+            if (f) HIPCHK(hipMemcpyAsync(c.f[c.fcur], f, bytes, hipMemcpyHostToDevice, c.stream));
+            if (g) HIPCHK(hipMemcpyAsync(c.gd[c.fcur], g, bytes, hipMemcpyHostToDevice, c.stream));
+            if (h) HIPCHK(hipMemcpyAsync(c.h[c.hcur], h, bytes, hipMemcpyHostToDevice, c.stream));
+        } else {
+            ensure_scratch(c);
+            auto enc = [&](const double* src, int is_f, void* dst) {
+                HIPCHK(hipMemcpyAsync(c.scratch, src, bytes, hipMemcpyHostToDevice, c.stream));
+                lbm::mx::fg_encode(c.stream, c.g, is_f, c.scratch, D<float>(c.fei), D<float>(dst));
+                HIPCHK(hipGetLastError());
+                sync(c);
+            };
+            if (f) enc(f, 1, c.f[c.fcur]);
+            if (g) enc(g, 0, c.gd[c.fcur]);
+            if (h) {
+                HIPCHK(hipMemcpyAsync(c.scratch, h, bytes, hipMemcpyHostToDevice, c.stream));
+                lbm::mx::to_f32(c.stream, static_cast<int64_t>(lbm::Q * n), c.scratch, D<float>(c.h[c.hcur]));
+                HIPCHK(hipGetLastError());
+            }
         }
+        if (h) c.hs = HState::Explicit;
         if (f || g) c.fg = first_step ? FG::First : FG::Local;
-        HIPCHK(hipStreamSynchronize(c.stream));
+        sync(c);
     });
 }
 
@@ -619,13 +765,13 @@ int lbm_step(void* ctx, int32_t n)
 int lbm_download(void* ctx, double* fei, double* rho, double* u, double* v, double* w, double* p)
 {
     return guarded(ctx, [&](Ctx& c) {
-        download_field(c, fei, c.fei);
-        download_field(c, rho, c.rho);
-        download_field(c, u, c.u);
-        download_field(c, v, c.v);
-        download_field(c, w, c.w);
-        download_field(c, p, c.p[c.pcur]);
-        HIPCHK(hipStreamSynchronize(c.stream));
+        get_field(c, fei, c.fei);
+        get_field(c, rho, c.rho);
+        get_field(c, u, c.u);
+        get_field(c, v, c.v);
+        get_field(c, w, c.w);
+        if (p) HIPCHK(hipMemcpyAsync(p, c.p[c.pc], c.n() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+        sync(c);
     });
 }
 
@@ -633,29 +779,42 @@ int lbm_download_dists(void* ctx, double* f, double* g, double* h)
 {
     return guarded(ctx, [&](Ctx& c) {
         const size_t bytes = lbm::Q * c.n() * sizeof(double);
-        if (!c.scratch) c.scratch = dalloc(lbm::Q * c.n());
-        auto get_fg = [&](double* out, double* const* bufs) {
+        ensure_scratch(c);
+        auto get_fg = [&](double* out, void* const* bufs, int is_f) {
             if (!out) return;
-            if (c.fg == FG::Post) {
-                lbm::f64::materialize(c.stream, c.g, bufs[c.fcur], c.scratch);
+            const int pulled = c.fg == FG::Post;
+            if (!c.mixed) {
+                if (pulled) {
+                    lbm::f64::materialize(c.stream, c.g, D<double>(bufs[c.fcur]), c.scratch);
+                    HIPCHK(hipGetLastError());
+                    HIPCHK(hipMemcpyAsync(out, c.scratch, bytes, hipMemcpyDeviceToHost, c.stream));
+                } else {
+                    HIPCHK(hipMemcpyAsync(out, bufs[c.fcur], bytes, hipMemcpyDeviceToHost, c.stream));
+                }
+            } else {
+                lbm::mx::fg_full(c.stream, c.g, pulled, is_f, D<float>(bufs[c.fcur]), D<float>(c.fei),
+                                 c.scratch);
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipMemcpyAsync(out, c.scratch, bytes, hipMemcpyDeviceToHost, c.stream));
-            } else {
-                HIPCHK(hipMemcpyAsync(out, bufs[c.fcur], bytes, hipMemcpyDeviceToHost, c.stream));
             }
-            HIPCHK(hipStreamSynchronize(c.stream));
+            sync(c);
         };
-        get_fg(f, c.f);
-        get_fg(g, c.gd);
+        get_fg(f, c.f, 1);
+        get_fg(g, c.gd, 0);
         if (h) {
             if (c.hs == HState::Equilibrium) {
-                lbm::f64::equilibrium_h(c.stream, c.g, c.p[c.pcur], c.scratch);
+                lbm::f64::equilibrium_h(c.stream, c.g, c.p[c.pc], c.scratch);
                 HIPCHK(hipGetLastError());
                 HIPCHK(hipMemcpyAsync(h, c.scratch, bytes, hipMemcpyDeviceToHost, c.stream));
-            } else {
+            } else if (!c.mixed) {
                 HIPCHK(hipMemcpyAsync(h, c.h[c.hcur], bytes, hipMemcpyDeviceToHost, c.stream));
+            } else {
+                lbm::mx::to_f64(c.stream, static_cast<int64_t>(lbm::Q * c.n()), D<float>(c.h[c.hcur]),
+                                c.scratch);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(h, c.scratch, bytes, hipMemcpyDeviceToHost, c.stream));
             }
-            HIPCHK(hipStreamSynchronize(c.stream));
+            sync(c);
         }
     });
 }
@@ -669,7 +828,7 @@ int lbm_poisson_diag(const void* ctx, lbm_poisson_diag_t* out)
 
 int lbm_synchronize(void* ctx)
 {
-    return guarded(ctx, [&](Ctx& c) { HIPCHK(hipStreamSynchronize(c.stream)); });
+    return guarded(ctx, [&](Ctx& c) { sync(c); });
 }
 
 int lbm_set_timing(void* ctx, int enabled)
@@ -693,10 +852,15 @@ int lbm_mass(void* ctx, double* sum_phi)
 {
     return guarded(ctx, [&](Ctx& c) {
         if (!sum_phi) throw ApiError{LBM_ERR_ARG, "null argument"};
-        lbm::f64::sum_field(c.stream, c.g, c.fei, c.partials, c.dst);
+        if (!c.mixed) {
+            lbm::f64::sum_field(c.stream, c.g, D<double>(c.fei), c.partials, c.dst);
+        } else {
+            lbm::mx::sum_f32(c.stream, c.g, D<float>(c.fei), c.partials);
+            lbm::f64::mass_finalize(c.stream, c.partials, lbm::red_blocks(c.g.n), c.dst);
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(c.hst, c.dst, sizeof(DevState), hipMemcpyDeviceToHost, c.stream));
-        HIPCHK(hipStreamSynchronize(c.stream));
+        sync(c);
         *sum_phi = c.hst->mass;
     });
 }

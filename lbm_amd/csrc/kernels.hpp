@@ -85,6 +85,13 @@ void check(hipStream_t s, const Grid& g, const Phys& ph, double* p, double* p_pr
            const double* hin, const double* tau, const double* src, int dual, double tol,
            double* partials, DevState* st);
 void streamed_to_post_boundary(hipStream_t s, const Grid& g, const double* hin, double* out);
+// FP64 reductions shared with the mixed-precision path.
+void gauge_finalize(hipStream_t s, const Phys& ph, const double* parts, int nparts, DevState* st);
+void gauge_error(hipStream_t s, const Grid& g, double* p, double* p_prev, const DevState* st,
+                 double* parts);
+void check_finalize(hipStream_t s, const double* parts_err, const double* parts_fp, int nparts,
+                    int dual, double tol, DevState* st);
+void mass_finalize(hipStream_t s, const double* parts, int nparts, DevState* st);
 void correct(hipStream_t s, const Grid& g, const double* p, const double* rho, double* u,
              double* v, double* w);
 void materialize(hipStream_t s, const Grid& g, const double* in, double* out);
@@ -92,5 +99,43 @@ void equilibrium_h(hipStream_t s, const Grid& g, const double* p, double* h);
 void sum_field(hipStream_t s, const Grid& g, const double* a, double* partials, DevState* st);
 
 } // namespace f64
+
+// Mixed precision: FP32 storage and collision arithmetic for f, g, h (as
+// deviations) and the macro fields; FP64 pressure, p_prev and every global sum.
+namespace mx {
+
+struct PhysF {
+    float rho_L, rho_G, mu_G, tauf_inv, taug_inv, k_f, k_g, T, a, b, fei_L, fei_G;
+    float tmp1, fei_avg, visc;  // visc = 1.5 * (taug - 0.5) * 2.0
+};
+
+void collide(hipStream_t s, int mode, const Grid& g, const PhysF& ph, const float* fei,
+             const float* rho, const float* u, const float* v, const float* w, const double* p,
+             const float* fin, const float* gin, float* fout, float* gout, float* ux, float* vy,
+             float* wz, double* partials);
+void macro(hipStream_t s, const Grid& g, const PhysF& ph, float* fpost, const float* gpost,
+           const float* ux, const float* vy, const float* wz, const DevState* st, float* fei,
+           float* rho, float* u, float* v, float* w, float* omega, float* src3);
+void poisson_iter(hipStream_t s, int mode, const Grid& g, const float* hin, float* hout,
+                  const float* bbase, const double* pown, float* bout, const float* omega,
+                  const float* src3, const DevState* st);
+void getp_sum(hipStream_t s, const Grid& g, const float* hin, const float* pbase, double* p,
+              double* parts);
+void fixed_point(hipStream_t s, const Grid& g, const float* hin, const float* pbase,
+                 const double* p, const float* omega, const float* src3, const DevState* st,
+                 double* parts);
+void correct(hipStream_t s, const Grid& g, const double* p, const float* rho, float* u, float* v,
+             float* w);
+void sum_f32(hipStream_t s, const Grid& g, const float* a, double* parts);
+void to_f64(hipStream_t s, int64_t n, const float* in, double* out);
+void to_f32(hipStream_t s, int64_t n, const double* in, float* out);
+// Distributions as the reference holds them after a step, in FP64.
+void fg_full(hipStream_t s, const Grid& g, int pulled, int is_f, const float* in, const float* fei,
+             double* out);
+// Uploaded post-boundary f or g (FP64) to the local deviation form.
+void fg_encode(hipStream_t s, const Grid& g, int is_f, const double* in, const float* fei,
+               float* out);
+
+} // namespace mx
 
 } // namespace lbm

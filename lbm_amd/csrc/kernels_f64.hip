@@ -704,6 +704,28 @@ void check(hipStream_t s, const Grid& g, const Phys& ph, double* p, double* p_pr
     k_check_finalize<<<1, BLOCK, 0, s>>>(parts_err, parts_fp, nb, dual, tol, st);
 }
 
+void gauge_finalize(hipStream_t s, const Phys& ph, const double* parts, int nparts, DevState* st)
+{
+    k_gauge_finalize<<<1, BLOCK, 0, s>>>(ph, parts, nparts, st);
+}
+
+void gauge_error(hipStream_t s, const Grid& g, double* p, double* p_prev, const DevState* st,
+                 double* parts)
+{
+    k_gauge_error<<<red_blocks(g.n), BLOCK, 0, s>>>(g, p, p_prev, st, parts);
+}
+
+void check_finalize(hipStream_t s, const double* parts_err, const double* parts_fp, int nparts,
+                    int dual, double tol, DevState* st)
+{
+    k_check_finalize<<<1, BLOCK, 0, s>>>(parts_err, parts_fp, nparts, dual, tol, st);
+}
+
+void mass_finalize(hipStream_t s, const double* parts, int nparts, DevState* st)
+{
+    k_mass_finalize<<<1, BLOCK, 0, s>>>(parts, nparts, st);
+}
+
 void streamed_to_post_boundary(hipStream_t s, const Grid& g, const double* hin, double* out)
 {
     k_streamed_view<<<grid_blocks(g.n), BLOCK, 0, s>>>(g, hin, out);

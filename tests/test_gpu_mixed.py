@@ -1,0 +1,80 @@
+"""Mixed precision (FP32 deviations + FP64 pressure and sums) against the
+FP64 oracle.
+
+Bar (north star): relative L-infinity error of phi, u and p <= 1e-5 against
+the FP64 reference; the same Poisson iteration counts on these cases.
+"""
+import numpy as np
+import pytest
+
+from lbm_amd import hip
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+TOL_MIXED = 1e-5
+FIELDS = ("fei", "rho", "u", "v", "w", "p")
+RATIO1000 = dict(rho_L=1000.0, rho_G=1.0, mu_L=0.008, mu_G=8e-5, k_g=1e-4, fei_max=0.093,
+                 fei_min=0.016, fei_L=0.093, fei_G=0.016)
+
+
+def rel_linf(a, b):
+    return float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300)
+
+
+def pair(P, init=None, **poisson):
+    o = po.Oracle(P)
+    if init is None:
+        o.init_two_droplets_sharp()
+    else:
+        o.place_droplet(*init, profile="tanh", width=2.0)
+    sim = hip.InamuroHIP(P, precision=hip.MIXED)
+    sim.upload_fields(o.fields())
+    if poisson:
+        o.set_poisson(**poisson)
+        sim.setPoisson(poisson.get("check_interval", 100), poisson.get("tol", 1e-3),
+                       poisson.get("iter_cap", 0), poisson.get("dual", False))
+    return o, sim
+
+
+@pytest.mark.parametrize("case", ["smoke", "ratio1000"])
+def test_mixed_tracks_the_fp64_oracle(case):
+    if case == "smoke":
+        P, init, steps = po.Params(lx=16, ly=32, lz=32, DD=8.0), None, 3
+    else:
+        P, init, steps = po.Params(lx=24, ly=24, lz=24, DD=12.0, **RATIO1000), (12.0, 12.0, 12.0), 3
+    o, sim = pair(P, init)
+    for s in range(steps):
+        gi, oi = sim.step(1), o.step(1)
+        assert gi == oi, (s + 1, gi, oi)
+        a, b = sim.downloadFieldsToCPU(), o.fields()
+        for k in FIELDS:
+            err = rel_linf(a[k], b[k])
+            assert err <= TOL_MIXED, (s + 1, k, err)
+    d = sim.download_dists()
+    for k in ("f", "g", "h"):
+        assert rel_linf(d[k], o.get(k)) <= TOL_MIXED, k
+
+
+def test_mixed_mass_and_dual_residual():
+    P = po.Params(lx=16, ly=32, lz=32, DD=8.0)
+    o, sim = pair(P, dual=True)
+    assert sim.step(2) == o.step(2)
+    gd, od = sim.getLastPoissonDiagnostics(), o.diag()
+    assert gd.fixed_point_evaluated == 1 and gd.converged == 1
+    assert abs(gd.fixed_point_h_relative - od.fixed_point_h_relative) <= 1e-3 * od.fixed_point_h_relative
+    m, ref = sim.mass(), float(np.sum(o.get("fei")))
+    assert abs(m - ref) <= 1e-6 * ref
+
+
+def test_mixed_upload_dists_round_trip():
+    P = po.Params(lx=12, ly=10, lz=14, DD=6.0)
+    o = po.Oracle(P)
+    o.init_two_droplets_sharp()
+    o.step(1)
+    sim = hip.InamuroHIP(P, precision=hip.MIXED)
+    sim.upload_fields(o.fields())
+    sim.upload_dists(f=o.get("f"), g=o.get("g"), h=None, first_step=False)
+    d = sim.download_dists(("f", "g"))
+    for k in ("f", "g"):
+        assert rel_linf(d[k], o.get(k)) <= 1e-6, k
