@@ -133,9 +133,11 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="static_droplet_256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run one short solve per precision (for rocprofv3 --pmc runs) and exit")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the time step is not sharded across GPUs in this "
@@ -147,6 +149,16 @@ def main(argv=None):
     wl = WORKLOADS[args.workload]
     L, R = wl["L"], wl["R"]
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    if args.traffic_probe:
+        # One step from the cold start per precision: the first solve runs 100
+        # Poisson iterations (one check), enough dispatches for counter runs.
+        for p in precs:
+            sim, _ = make_sim(hip, L, R, p)
+            sim.performTimeStepGPU(1)
+            sim.synchronize()
+            print(f"traffic probe {p}: {sim.getLastPoissonDiagnostics().iterations} iterations", flush=True)
+            sim.close()
+        return 0
     results = {p: run_precision(hip, L, R, p, args.steps, args.warmup) for p in precs}
     head = results["mixed"] if "mixed" in results else results["fp64"]
     n_it_mean = sum(head["iterations"]) / len(head["iterations"])
@@ -189,9 +201,17 @@ def main(argv=None):
         line["fp64_roofline_frac"] = results["fp64"]["achieved_gbs"] / HBM_PEAK_GBS
         line["mixed_over_fp64_mlups"] = results["mixed"]["mlups"] / results["fp64"]["mlups"]
         line["mixed_over_fp64_iteration_time"] = results["fp64"]["poisson_iter_ms"] / results["mixed"]["poisson_iter_ms"]
-    traffic = os.path.join(ROOT, "profiles", f"traffic_{head['precision']}_{args.workload}.json")
-    if os.path.exists(traffic):
-        line["roofline"]["traffic"] = json.load(open(traffic)).get("bytes_per_launch")
+    # HBM bytes per launch of the dominant kernel, from the committed
+    # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy-kernel
+    # calibration); null when no measurement exists for this workload.
+    for prec in results:
+        traffic = os.path.join(ROOT, "profiles", f"traffic_{prec}_{args.workload}.json")
+        if os.path.exists(traffic):
+            tb = json.load(open(traffic)).get("bytes_per_launch")
+            if prec == head["precision"]:
+                line["roofline"]["traffic"] = tb
+            else:
+                line[f"{prec}_traffic_bytes_per_launch"] = tb
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(L, R, n_it_mean)
     print(json.dumps(line), flush=True)

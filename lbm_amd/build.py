@@ -51,6 +51,13 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    # Calibration program for the memory-traffic counters (not linked into
+    # the library).
+    probe = os.path.join(HERE, "copy_probe")
+    cmd = [HIPCC] + COMMON + [os.path.join(CSRC, "copy_probe.hip"), "-o", probe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for copy_probe.hip:\n{r.stderr}")
     if verbose:
         print(f"built {LIB}")
     return LIB

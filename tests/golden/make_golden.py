@@ -100,7 +100,9 @@ def make_init(case, P: po.Params, tmp: str):
     return fields, ipath
 
 
-def run_case(name: str, case: dict) -> None:
+def run_case(name: str, case: dict, reuse: str = None) -> None:
+    """reuse: an existing ref_probe output file for this case (same
+    parameters, initial fields and step count), to skip a long probe run."""
     P = po.Params(**case["params"])
     steps = case["steps"]
     with tempfile.TemporaryDirectory() as tmp:
@@ -108,11 +110,15 @@ def run_case(name: str, case: dict) -> None:
         pfile = os.path.join(tmp, "params.in")
         with open(pfile, "w") as fh:
             fh.write(P.params_in())
-        out = os.path.join(tmp, "out.bin")
+        out = reuse or os.path.join(tmp, "out.bin")
         t0 = time.time()
-        subprocess.run([po.REF_PROBE, pfile, init_arg, str(steps), out, "0"], check=True)
+        if not reuse:
+            subprocess.run([po.REF_PROBE, pfile, init_arg, str(steps), out, "0"], check=True)
         t_ref = time.time() - t0
         iters, step_ms, ref = po.read_probe_output(out)
+        if reuse:
+            t_ref = float(sum(step_ms)) / 1e3
+            assert len(iters) == steps, "reused probe output has a different step count"
 
     o = po.Oracle(P, threads=1)
     o.set("fei", init_fields["fei"]); o.set("rho", init_fields["rho"])
@@ -170,9 +176,15 @@ def run_case(name: str, case: dict) -> None:
 def main(argv):
     if not po.ref_available():
         subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "all", "ref"], check=True)
-    names = argv[1:] or [k for k in CASES if k != "config0_64"]
+    args = argv[1:]
+    reuse = None
+    if "--reuse" in args:
+        i = args.index("--reuse")
+        reuse = args[i + 1]
+        del args[i:i + 2]
+    names = args or [k for k in CASES if k != "config0_64"]
     for name in names:
-        run_case(name, CASES[name])
+        run_case(name, CASES[name], reuse)
 
 
 if __name__ == "__main__":
