@@ -150,8 +150,11 @@ int lbm_upload_dists(void* ctx, const double* f, const double* g, const double* 
                      int first_step);
 
 /* setPoissonConvergence / setPoissonIterationLimit / setUsePoissonDualResidual
- * (GPU/InamuroCUDA.cu:2573-2608).  iteration_limit = 0 is the unbounded
- * contract; a nonzero limit must be a multiple of check_interval. */
+ * / setEnablePoissonFixedPointShadow (GPU/InamuroCUDA.cu:2565-2608).
+ * iteration_limit = 0 is the unbounded contract; a nonzero limit must be a
+ * multiple of check_interval.  dual_residual: 0 off, 1 exit requires both the
+ * relative pressure change and the kinetic fixed-point residual below the
+ * tolerance, 2 evaluate the fixed-point residual without changing the exit. */
 int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance,
                     uint64_t iteration_limit, int dual_residual);
 

@@ -58,9 +58,33 @@ def build(verbose: bool = False) -> str:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for copy_probe.hip:\n{r.stderr}")
+    build_host()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+HOST = os.path.join(HERE, "host")
+HOST_SOURCES = ["InamuroHIP.cpp", "InamuroSolver.cpp"]
+
+
+def build_host() -> None:
+    """The C++ host layer (lbm_amd/host) and the lbm_gpu runner, built with
+    g++ against the C ABI only (no HIP headers needed)."""
+    cxx = os.environ.get("CXX", "g++")
+    flags = ["-std=c++17", "-O2", "-Wall", "-Wextra", "-fPIC", f"-I{os.path.dirname(HERE)}/include"]
+    link = [f"-L{HERE}", "-llbm_hip", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"]
+    srcs = [os.path.join(HOST, f) for f in HOST_SOURCES]
+    lib = os.path.join(HERE, "liblbm_host.so")
+    cmds = [
+        [cxx] + flags + ["-shared"] + srcs + ["-o", lib] + link,
+        [cxx] + flags + [os.path.join(HOST, "lbm_gpu.cpp")] + srcs + ["-o", os.path.join(HERE, "lbm_gpu")] + link,
+        [cxx] + flags + [os.path.join(HOST, "lbm_sim.cpp")] + srcs + ["-o", os.path.join(HERE, "lbm_sim")] + link,
+    ]
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"host build failed:\n{' '.join(cmd)}\n{r.stderr}")
 
 
 if __name__ == "__main__":

@@ -407,7 +407,7 @@ void step_one(Ctx& c)
     d = lbm_poisson_diag_t{};
     d.prev_iterations = prev;
     d.pressure_solve_call_count = c.solve_calls;
-    d.dual_residual_enabled = c.dual;
+    d.dual_residual_enabled = c.dual == 1;
     const double nan = std::numeric_limits<double>::quiet_NaN();
     d.fixed_point_h_l1 = d.fixed_point_h_scale = d.fixed_point_h_relative = nan;
     d.fixed_point_p_l1 = d.fixed_point_p_scale = d.fixed_point_p_relative = nan;
@@ -746,7 +746,9 @@ int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance, uint64_
         c.check_interval = check_interval;
         c.tol = tolerance;
         c.iter_limit = iteration_limit;
-        c.dual = dual_residual ? 1 : 0;
+        if (dual_residual < 0 || dual_residual > 2)
+            throw ApiError{LBM_ERR_ARG, "dual_residual must be 0 (off), 1 (dual exit) or 2 (shadow)"};
+        c.dual = dual_residual;
     });
 }
 

@@ -546,8 +546,10 @@ __global__ __launch_bounds__(BLOCK) void k_check_finalize(const double* __restri
         st->fp_rel = st->fp_h_rel > st->fp_p_rel ? st->fp_h_rel : st->fp_p_rel;
         if (!isfinite(st->fp_rel) && st->status == ST_OK) st->status = ST_FP_REL_NONFINITE;
         st->fp_evaluated = 1;
-        fp_ok = st->fp_rel < tol;
-        st->fp_converged = fp_ok;
+        st->fp_converged = st->fp_rel < tol;
+        // dual == 1: the exit also needs r_fp < tol; dual == 2: shadow only
+        // (GPU/main.cpp:185-190).
+        if (dual == 1) fp_ok = st->fp_converged;
     }
     st->e1 = e1;
     st->e2 = e2;

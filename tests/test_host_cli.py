@@ -1,0 +1,67 @@
+"""The C++ host layer: lbm_gpu flag validation (the reference runner's
+contract checks, GPU/main.cpp:202-247) and params.in handling.  These run
+without a GPU: every rejection happens before a device is touched."""
+import os
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LBM_GPU = os.path.join(ROOT, "lbm_amd", "lbm_gpu")
+LBM_SIM = os.path.join(ROOT, "lbm_amd", "lbm_sim")
+
+
+def run(args, cwd=None):
+    return subprocess.run([LBM_GPU] + args, capture_output=True, text=True, cwd=cwd, timeout=60)
+
+
+@pytest.mark.parametrize("args,msg", [
+    (["--poisson-check-interval", "50"], "strict production requires --poisson-check-interval 100"),
+    (["--poisson-tolerance", "1e-6"], "strict production requires --poisson-tolerance 1e-3"),
+    (["--poisson-iteration-limit", "500"], "strict production requires --poisson-iteration-limit 0 (unbounded)"),
+    (["--mode", "tpu"], "--mode must be cpu or gpu"),
+    (["--poisson", "magic"], "--poisson must be split, fused, onepass, or scalar"),
+    (["--pressure-boundary", "x"], "--pressure-boundary must be split or fused"),
+    (["--poisson-fixed-point-relax", "0"], "--poisson-fixed-point-relax must be positive"),
+    (["--poisson-anderson-beta-max", "-1"], "--poisson-anderson-beta-max must be non-negative"),
+    (["--poisson", "onepass"], "only the strict split+split Poisson contract"),
+    (["--poisson-anderson-m1"], "--poisson-anderson-m1 is a Poisson accelerator variant"),
+    (["--bogus"], "unknown argument: --bogus"),
+    (["--steps"], "missing value for --steps"),
+    (["--output-steps", "3,-1"], "--output-steps values must be non-negative"),
+    (["--precision", "fp16"], "--precision must be fp64 or mixed"),
+])
+def test_runner_rejects_non_contract_flags(args, msg):
+    r = run(args)
+    assert r.returncode == 1
+    assert msg in r.stderr
+
+
+def test_runner_reports_missing_parameter_file(tmp_path):
+    r = run(["--params", str(tmp_path / "nope.in")])
+    assert r.returncode == 1
+    assert "cannot open parameter file" in r.stderr
+
+
+def test_runner_rejects_malformed_parameter_file(tmp_path):
+    p = tmp_path / "bad.in"
+    p.write_text("16 32 32\n1\n20\n")
+    r = run(["--params", str(p)])
+    assert r.returncode == 1
+    assert "malformed parameter file" in r.stderr
+
+
+def test_solver_rejects_unknown_precision_key(tmp_path):
+    from oracle import pyoracle as po
+    p = tmp_path / "p.in"
+    p.write_text(po.Params(lx=8, ly=8, lz=8).params_in("precision=fp8\n"))
+    r = subprocess.run([LBM_SIM, str(p)], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1
+    assert "precision must be fp64 or mixed" in r.stderr
+
+
+def test_help_lists_the_contract_flags():
+    r = run(["--help"])
+    assert r.returncode == 0
+    for flag in ("--poisson-check-interval 100", "--poisson-tolerance 1e-3", "--poisson-dual-residual"):
+        assert flag in r.stdout
