@@ -158,6 +158,18 @@ int lbm_upload_dists(void* ctx, const double* f, const double* g, const double* 
 int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance,
                     uint64_t iteration_limit, int dual_residual);
 
+/* Frozen Poisson iteration counts (no counterpart in the reference): step i
+ * after this call runs exactly iterations[i] iterations, gauging at every
+ * check, whatever the residual; later steps converge as usual.  This makes the
+ * step a smooth map of its inputs (the stopping test is not), which is what
+ * finite-difference checks of the adjoint need.  count = 0 clears it. */
+int lbm_set_iteration_schedule(void* ctx, const uint64_t* iterations, int64_t count);
+
+/* Poisson iterations of every step since the last upload (the CPU solver's
+ * per-step pressure_iterations, CPU/InamuroSolver.cpp:318-324); *count gets
+ * the number of steps, at most `capacity` entries are written. */
+int lbm_iteration_history(const void* ctx, uint64_t* out, int64_t capacity, int64_t* count);
+
 /* performTimeStepGPU (GPU/InamuroCUDA.cu:2372-2442), n times. */
 int lbm_step(void* ctx, int32_t n);
 

@@ -98,6 +98,9 @@ struct Ctx {
     lbm_poisson_diag_t diag{};
     uint64_t last_iterations = 0;
     int32_t solve_calls = 0;
+    std::vector<uint64_t> history;   // iterations of every step since the last upload
+    std::vector<uint64_t> schedule;  // frozen per-step iteration counts (empty = converge)
+    size_t sched_pos = 0;
 
     bool timing = false;
     lbm_perf_t perf{};
@@ -199,6 +202,8 @@ void reset_state(Ctx& c)
     c.last_iterations = 0;
     c.solve_calls = 0;
     c.diag = lbm_poisson_diag_t{};
+    c.history.clear();
+    c.sched_pos = 0;
 }
 
 void check_params(const lbm_params_t& P)
@@ -414,6 +419,7 @@ void step_one(Ctx& c)
     d.fixed_point_relative = nan;
 
     int hmode = c.hs == HState::Equilibrium ? lbm::H_EQUILIBRIUM : lbm::H_EXPLICIT;
+    const uint64_t forced = c.sched_pos < c.schedule.size() ? c.schedule[c.sched_pos] : 0;
     uint64_t iter = 0;
     bool first_segment = true;
     for (;;) {
@@ -466,6 +472,11 @@ void step_one(Ctx& c)
         }
         d.converged = st.converged;
         if (st.status != lbm::ST_OK) throw ApiError{LBM_ERR_NUMERIC, status_message(st.status)};
+        if (forced != 0) {
+            if (iter >= forced) break;
+            hmode = lbm::H_GAUGED;
+            continue;
+        }
         if (st.converged) break;
         if (c.iter_limit != 0 && iter >= c.iter_limit) {
             d.hit_max_iterations = 1;
@@ -475,6 +486,8 @@ void step_one(Ctx& c)
     }
     c.last_iterations = iter;
     c.perf.poisson_iterations += iter;
+    c.history.push_back(iter);
+    if (forced != 0) ++c.sched_pos;
 
     // Velocity correction; h = E*p is implied for the next step.
     record(c, 7);
@@ -750,6 +763,28 @@ int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance, uint64_
             throw ApiError{LBM_ERR_ARG, "dual_residual must be 0 (off), 1 (dual exit) or 2 (shadow)"};
         c.dual = dual_residual;
     });
+}
+
+int lbm_set_iteration_schedule(void* ctx, const uint64_t* iterations, int64_t count)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        if (count < 0 || (count > 0 && !iterations)) throw ApiError{LBM_ERR_ARG, "bad iteration schedule"};
+        for (int64_t i = 0; i < count; ++i)
+            if (iterations[i] == 0 || iterations[i] % static_cast<uint64_t>(c.check_interval) != 0)
+                throw ApiError{LBM_ERR_ARG, "scheduled iteration counts must be positive multiples of the check interval"};
+        c.schedule.assign(iterations, iterations + count);
+        c.sched_pos = 0;
+    });
+}
+
+int lbm_iteration_history(const void* ctx, uint64_t* out, int64_t capacity, int64_t* count)
+{
+    if (!ctx || !count || capacity < 0 || (capacity > 0 && !out)) return LBM_ERR_ARG;
+    const Ctx& c = *static_cast<const Ctx*>(ctx);
+    *count = static_cast<int64_t>(c.history.size());
+    const int64_t m = std::min<int64_t>(capacity, *count);
+    for (int64_t i = 0; i < m; ++i) out[i] = c.history[static_cast<size_t>(i)];
+    return LBM_OK;
 }
 
 int lbm_step(void* ctx, int32_t n)

@@ -41,7 +41,7 @@ EXPORTED = (
     "lbm_set_poisson", "lbm_step", "lbm_download", "lbm_download_dists",
     "lbm_poisson_diag", "lbm_synchronize", "lbm_set_timing", "lbm_perf_get",
     "lbm_perf_reset", "lbm_mass", "lbm_init_reference", "lbm_place_droplet",
-    "lbm_reset_fields",
+    "lbm_reset_fields", "lbm_set_iteration_schedule", "lbm_iteration_history",
 )
 
 
@@ -114,6 +114,9 @@ def load_library():
     L.lbm_init_reference.argtypes = [vp]
     L.lbm_reset_fields.argtypes = [vp]
     L.lbm_place_droplet.argtypes = [vp] + [ctypes.c_double] * 6 + [ctypes.c_int, ctypes.c_double]
+    L.lbm_set_iteration_schedule.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64]
+    L.lbm_iteration_history.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64,
+                                        ctypes.POINTER(ctypes.c_int64)]
     _lib = L
     return L
 
@@ -267,6 +270,20 @@ class InamuroHIP:
     def setUsePoissonDualResidual(self, enabled: bool):
         ci, tol, lim, _ = self._cfg()
         self.setPoisson(ci, tol, lim, enabled)
+
+    def set_iteration_schedule(self, iterations):
+        """Freeze the Poisson iteration count of the next len(iterations) steps."""
+        arr = np.ascontiguousarray(np.asarray(list(iterations), dtype=np.uint64))
+        self._check(load_library().lbm_set_iteration_schedule(
+            self._h, arr.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(arr)))
+
+    def iteration_history(self) -> list:
+        n = ctypes.c_int64()
+        self._check(load_library().lbm_iteration_history(self._h, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), dtype=np.uint64)
+        self._check(load_library().lbm_iteration_history(
+            self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), n.value, ctypes.byref(n)))
+        return [int(x) for x in out[:n.value]]
 
     # -- stepping -----------------------------------------------------------
     def performTimeStepGPU(self, n: int = 1):

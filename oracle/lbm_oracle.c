@@ -71,6 +71,8 @@ struct orc_sim {
     int dual;
     uint64_t last_iteration_count;
     int pressure_solve_call_count;
+    uint64_t* schedule;                 /* frozen per-step iteration counts (extension) */
+    int schedule_len, schedule_pos;
     orc_poisson_diag diag;
     char err[256];
 };
@@ -153,6 +155,7 @@ void orc_destroy(orc_sim* s)
     free(s->f); free(s->g); free(s->h); free(s->tmp);
     for (int i = 0; i < ORC_NUM_SCALAR_ARRAYS; ++i) free(A(s, i));
     free(s->xn); free(s->yn); free(s->zn);
+    free(s->schedule);
     free(s);
 }
 
@@ -646,7 +649,13 @@ void orc_poisson_iterations(orc_sim* s, int k)
 /* Inamuro::solvePressurePoisson, CPU/Inamuro.cpp:624-724.  p_prev is the
  * reference's function-static pressure_prev, zero-initialised once per
  * simulation and kept across steps. */
-static int solve_pressure_poisson(orc_sim* s)
+static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p);
+
+static int solve_pressure_poisson(orc_sim* s) { return solve_pressure_poisson_rec(s, NULL, NULL); }
+
+/* rec_h / rec_p (extension, for the adjoint): when non-NULL, receive the h and
+ * p each iteration starts from (after any gauge), iteration-major. */
+static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p)
 {
     const size_t n = s->n;
     ++s->pressure_solve_call_count;
@@ -679,9 +688,14 @@ static int solve_pressure_poisson(orc_sim* s)
     }
     for (size_t i = 0; i < n; ++i) ux[i] -= divergence_mean;
 
+    const uint64_t forced = s->schedule_pos < s->schedule_len ? s->schedule[s->schedule_pos] : 0;
     uint64_t iter = 0;
     for (;;) {
         if (iter == UINT64_MAX) { set_err(s, "Poisson iteration counter overflow"); return -2; }
+        if (rec_h) {
+            memcpy(rec_h + iter * (size_t)Q * n, s->h, sizeof(double) * (size_t)Q * n);
+            memcpy(rec_p + iter * n, p, sizeof(double) * n);
+        }
         ++iter;
         collision_p_on(s, s->h);
         stream(s, s->h);
@@ -707,12 +721,37 @@ static int solve_pressure_poisson(orc_sim* s)
             if (!isfinite(error)) { set_err(s, "Poisson residual is NaN/Inf"); return -1; }
             d->relative_error = error;
             d->iteration_count = iter;
+            if (forced) {
+                if (iter >= forced) break;
+                continue;
+            }
             if (error < s->tol && fp_ok) { d->converged = 1; break; }
             if (s->iter_cap != 0 && iter >= s->iter_cap) { d->hit_max_iterations = 1; break; }
         }
     }
     s->last_iteration_count = iter;
     d->iteration_count = iter;
+    if (forced) ++s->schedule_pos;
+    return 0;
+}
+
+/* Extension: freeze the iteration count of the next `count` solves. */
+int orc_set_schedule(orc_sim* s, const uint64_t* iters, int count)
+{
+    free(s->schedule);
+    s->schedule = NULL;
+    s->schedule_len = s->schedule_pos = 0;
+    if (count <= 0) return 0;
+    s->schedule = (uint64_t*)malloc(sizeof(uint64_t) * (size_t)count);
+    if (!s->schedule) { set_err(s, "out of memory"); return -1; }
+    for (int i = 0; i < count; ++i) {
+        if (iters[i] == 0 || iters[i] % s->check_interval != 0) {
+            set_err(s, "scheduled iteration counts must be positive multiples of the check interval");
+            return -1;
+        }
+        s->schedule[i] = iters[i];
+    }
+    s->schedule_len = count;
     return 0;
 }
 
@@ -788,3 +827,444 @@ int orc_step(orc_sim* s)
 }
 
 void orc_poisson_diag_get(const orc_sim* s, orc_poisson_diag* out) { *out = s->diag; }
+
+
+/* ================================================================== */
+/* Discrete adjoint of the frozen-schedule step (extension)           */
+/* ================================================================== */
+/*
+ * The reference has no adjoint.  Below is the reverse mode of the step above
+ * with every Poisson solve's iteration count frozen (orc_set_schedule): each
+ * primitive is transposed by hand -- correct_uvw, every executed Poisson
+ * iteration with its gauge shifts and the source-mean projection, getMacro
+ * with the clamped density map, slip and stream, the BGK collision with its
+ * first-step equilibrium start, and the gradient / Laplacian stencils.  The
+ * forward pass keeps each step's start state; the reverse pass recomputes one
+ * step at a time from it, recording that step's Poisson iterates, so memory
+ * holds one step's iterates (small grids: this is a checker).
+ *
+ * Objective J = sum (phi_T - target)^2 after `steps` steps from phi0, with
+ * rho0 = rho_G + (phi0 - fei_min)(rho_L - rho_G)/(fei_max - fei_min) (the
+ * density rule of place_droplet, GPU/Inamuro.cpp:521-527), zero u, v, w, p,
+ * h = 0 and the first-step flag set.
+ */
+
+typedef struct {
+    double* m[6];     /* step-start fei, rho, u, v, w, p */
+    double* p_prev;   /* step-start pressure_prev (only feeds the residual) */
+    double* f;        /* step-start streamed f, g (unused on the first step) */
+    double* g;
+    int first;
+} adj_ckpt;
+
+static void load_canonical(orc_sim* s, int id_scratch, const double* canon, double* internal)
+{
+    double* save = A(s, id_scratch);
+    A(s, id_scratch) = internal;
+    orc_set(s, id_scratch, canon);
+    A(s, id_scratch) = save;
+}
+
+static void store_canonical(const orc_sim* s, int id_scratch, const double* internal, double* canon)
+{
+    orc_sim* w = (orc_sim*)s;
+    double* save = A(w, id_scratch);
+    A(w, id_scratch) = (double*)internal;
+    orc_get(w, id_scratch, canon);
+    A(w, id_scratch) = save;
+}
+
+static void adj_reset(orc_sim* s, const double* phi0_canon)
+{
+    const orc_params* P = &s->P;
+    const size_t n = s->n;
+    orc_set(s, ORC_FEI, phi0_canon);
+    for (size_t i = 0; i < n; ++i) {
+        A(s, ORC_RHO)[i] =
+            P->rho_G + (A(s, ORC_FEI)[i] - P->fei_min) * (P->rho_L - P->rho_G) / (P->fei_max - P->fei_min);
+        A(s, ORC_U)[i] = A(s, ORC_V)[i] = A(s, ORC_W)[i] = 0.0;
+        A(s, ORC_P)[i] = A(s, ORC_P_PREV)[i] = 0.0;
+    }
+    memset(s->f, 0, sizeof(double) * (size_t)Q * n);
+    memset(s->g, 0, sizeof(double) * (size_t)Q * n);
+    memset(s->h, 0, sizeof(double) * (size_t)Q * n);
+    s->first_step = 1;
+    s->last_iteration_count = 0;
+    s->pressure_solve_call_count = 0;
+}
+
+static double adj_misfit(const orc_sim* s, const double* target)
+{
+    double J = 0.0;
+    for (size_t i = 0; i < s->n; ++i) {
+        const double d = A(s, ORC_FEI)[i] - target[i];
+        J += d * d;
+    }
+    return J;
+}
+
+int orc_objective(orc_sim* s, const double* phi0, const double* target, int steps,
+                  const uint64_t* iters, double* J)
+{
+    double* t = (double*)malloc(sizeof(double) * s->n);
+    if (!t) { set_err(s, "out of memory"); return -1; }
+    load_canonical(s, ORC_P_PREV, target, t);
+    adj_reset(s, phi0);
+    int rc = orc_set_schedule(s, iters, steps);
+    for (int k = 0; rc == 0 && k < steps; ++k) rc = orc_step(s);
+    if (rc == 0) *J = adj_misfit(s, t);
+    orc_set_schedule(s, NULL, 0);
+    free(t);
+    return rc;
+}
+
+/* Transpose of stream + slipBounceBack (CPU/Inamuro.cpp:252-292, 589-623) on
+ * a [q][n] adjoint array, in place. */
+static void adj_stream_slip(orc_sim* s, double* lam)
+{
+    const size_t n = s->n;
+    const int lx = s->lx, ly = s->ly, lz = s->lz;
+    /* slip wrote dst <- src: src += dst, dst = 0 (right wall last, so first) */
+    for (int y = 0; y < ly; ++y)
+        for (int z = 0; z < lz; ++z)
+            for (int r = 0; r < 5; ++r) {
+                const size_t ir = IX(s, lx - 1, y, z);
+                lam[(size_t)SLIP_POS[r] * n + ir] += lam[(size_t)SLIP_NEG[r] * n + ir];
+                lam[(size_t)SLIP_NEG[r] * n + ir] = 0.0;
+            }
+    for (int y = 0; y < ly; ++y)
+        for (int z = 0; z < lz; ++z)
+            for (int r = 0; r < 5; ++r) {
+                const size_t il = IX(s, 0, y, z);
+                lam[(size_t)SLIP_NEG[r] * n + il] += lam[(size_t)SLIP_POS[r] * n + il];
+                lam[(size_t)SLIP_POS[r] * n + il] = 0.0;
+            }
+    /* stream wrote out_q(x) = in_q(x - c_q): in_q(y) = out_q(y + c_q) */
+    for (int q = 0; q < Q; ++q) {
+        const double* src = lam + (size_t)q * n;
+        double* dst = s->tmp + (size_t)q * n;
+        for (int x = 0; x < lx; ++x) {
+            const int xs = (x + EX[q] + lx) % lx;
+            for (int y = 0; y < ly; ++y) {
+                const int ys = (y + EY[q] + ly) % ly;
+                for (int z = 0; z < lz; ++z) dst[IX(s, x, y, z)] = src[IX(s, xs, ys, (z + EZ[q] + lz) % lz)];
+            }
+        }
+    }
+    memcpy(lam, s->tmp, sizeof(double) * (size_t)Q * n);
+}
+
+/* Transpose of firstord (gradient adjoints gx, gy, gz) and secondord
+ * (Laplacian adjoint lp), CPU/Inamuro.hpp:213-314; any may be NULL.
+ * Accumulates into out. */
+static void adj_stencil(orc_sim* s, const double* gx, const double* gy, const double* gz,
+                        const double* lp, double* out)
+{
+    for (int x = 0; x < s->lx; ++x)
+        for (int y = 0; y < s->ly; ++y)
+            for (int z = 0; z < s->lz; ++z) {
+                const size_t i = IX(s, x, y, z);
+                const double ax = gx ? gx[i] / 10.0 : 0.0;
+                const double ay = gy ? gy[i] / 10.0 : 0.0;
+                const double az = gz ? gz[i] / 10.0 : 0.0;
+                const double al = lp ? lp[i] / 5.0 : 0.0;
+                for (int k = 1; k < Q; ++k) {
+                    const size_t j = IX(s, s->xn[x * 3 + EX[k] + 1], s->yn[y * 3 + EY[k] + 1],
+                                        s->zn[z * 3 + EZ[k] + 1]);
+                    out[j] += UC[k] * ax + VC[k] * ay + WC[k] * az + al;
+                }
+                out[i] -= 14.0 * al;
+            }
+}
+
+/* d rho / d fei of the clamped sine map (CPU/Inamuro.cpp:313-327); zero on
+ * the clamped branches, matching their one-sided derivative. */
+static double drho_dfei(const orc_params* P, double fei)
+{
+    if (fei <= P->fei_G || fei >= P->fei_L) return 0.0;
+    const double k = M_PI / (P->fei_L - P->fei_G);
+    const double arg = (fei - (P->fei_L + P->fei_G) / 2.0) * k;
+    return (P->rho_L - P->rho_G) / 2.0 * cos(arg) * k;
+}
+
+/* Gradient-array slots of the collision adjoint. */
+enum { GA_UX, GA_UY, GA_UZ, GA_VX, GA_VY, GA_VZ, GA_WX, GA_WY, GA_WZ,
+       GA_RX, GA_RY, GA_RZ, GA_FX, GA_FY, GA_FZ, GA_ULAP, GA_VLAP, GA_WLAP, GA_FLAP, GA_NUM };
+
+int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
+                const uint64_t* iters, double* J, double* grad_phi0, double* grad_kg)
+{
+    const orc_params* P = &s->P;
+    const size_t n = s->n;
+    const size_t qn = (size_t)Q * n;
+    const int lx = s->lx, ly = s->ly, lz = s->lz;
+    int rc = -1;
+    uint64_t max_it = 0;
+    for (int k = 0; k < steps; ++k) max_it = iters[k] > max_it ? iters[k] : max_it;
+
+    adj_ckpt* ck = (adj_ckpt*)calloc((size_t)(steps > 0 ? steps : 1), sizeof(adj_ckpt));
+    double* tgt = (double*)malloc(sizeof(double) * n);
+    double* rec_h = (double*)malloc(sizeof(double) * qn * (size_t)(max_it ? max_it : 1));
+    double* rec_p = (double*)malloc(sizeof(double) * n * (size_t)(max_it ? max_it : 1));
+    double* fei_new = (double*)malloc(sizeof(double) * n);
+    double* rho_new = (double*)malloc(sizeof(double) * n);
+    double* tau = (double*)malloc(sizeof(double) * n);
+    double* pfin = (double*)malloc(sizeof(double) * n);
+    double* lf = (double*)calloc(qn, sizeof(double));   /* adjoint of f (then of collision output) */
+    double* lg = (double*)calloc(qn, sizeof(double));
+    double* cf = (double*)calloc(qn, sizeof(double));   /* adjoint of the streamed f, g a step reads */
+    double* cg = (double*)calloc(qn, sizeof(double));
+    double* lh = (double*)calloc(qn, sizeof(double));
+    double* lp = (double*)calloc(n, sizeof(double));
+    double* ltau = (double*)calloc(n, sizeof(double));
+    double* lS = (double*)calloc(n, sizeof(double));
+    double* la[6] = {0, 0, 0, 0, 0, 0};               /* adjoints of fei, rho, u, v, w, p */
+    double* ga[GA_NUM];
+    double kg_bar = 0.0;
+    int ok = ck && tgt && rec_h && rec_p && fei_new && rho_new && tau && pfin && lf && lg && cf && cg && lh &&
+             lp && ltau && lS;
+    for (int k = 0; k < 6; ++k) { la[k] = (double*)calloc(n, sizeof(double)); ok = ok && la[k]; }
+    for (int k = 0; k < GA_NUM; ++k) { ga[k] = (double*)calloc(n, sizeof(double)); ok = ok && ga[k]; }
+    for (int k = 0; ok && k < steps; ++k) {
+        for (int j = 0; j < 6; ++j) { ck[k].m[j] = (double*)malloc(sizeof(double) * n); ok = ok && ck[k].m[j]; }
+        ck[k].f = (double*)malloc(sizeof(double) * qn);
+        ck[k].g = (double*)malloc(sizeof(double) * qn);
+        ck[k].p_prev = (double*)malloc(sizeof(double) * n);
+        ok = ok && ck[k].f && ck[k].g && ck[k].p_prev;
+    }
+    if (!ok) { set_err(s, "out of memory"); goto done; }
+    if (orc_set_schedule(s, iters, steps) != 0) goto done;
+
+    /* ---- forward, keeping every step's start state ---- */
+    load_canonical(s, ORC_P_PREV, target, tgt);
+    adj_reset(s, phi0);
+    for (int k = 0; k < steps; ++k) {
+        for (int j = 0; j < 6; ++j) memcpy(ck[k].m[j], A(s, ORC_FEI + j), sizeof(double) * n);
+        memcpy(ck[k].p_prev, A(s, ORC_P_PREV), sizeof(double) * n);
+        memcpy(ck[k].f, s->f, sizeof(double) * qn);
+        memcpy(ck[k].g, s->g, sizeof(double) * qn);
+        ck[k].first = s->first_step;
+        if (orc_step(s) != 0) goto done;
+    }
+    *J = adj_misfit(s, tgt);
+    for (size_t i = 0; i < n; ++i) la[0][i] = 2.0 * (A(s, ORC_FEI)[i] - tgt[i]);
+
+    /* ---- reverse ---- */
+    for (int k = steps - 1; k >= 0; --k) {
+        const adj_ckpt* c = &ck[k];
+        const uint64_t nit = iters[k];
+        /* recompute step k up to the end of its Poisson solve */
+        for (int j = 0; j < 6; ++j) memcpy(A(s, ORC_FEI + j), c->m[j], sizeof(double) * n);
+        memcpy(A(s, ORC_P_PREV), c->p_prev, sizeof(double) * n);
+        memcpy(s->f, c->f, sizeof(double) * qn);
+        memcpy(s->g, c->g, sizeof(double) * qn);
+        s->first_step = c->first;
+        if (c->first) memset(s->h, 0, sizeof(double) * qn);
+        else update_hh(s);
+        orc_set_schedule(s, &iters[k], 1);
+        collision(s);
+        stream(s, s->f); stream(s, s->g);
+        slip_bounce_back(s, s->f); slip_bounce_back(s, s->g);
+        get_macro(s);
+        memcpy(fei_new, A(s, ORC_FEI), sizeof(double) * n);
+        memcpy(rho_new, A(s, ORC_RHO), sizeof(double) * n);
+        for (size_t i = 0; i < n; ++i) tau[i] = 1.0 / rho_new[i] + 0.5;
+        if (solve_pressure_poisson_rec(s, rec_h, rec_p) != 0) goto done;
+        memcpy(pfin, A(s, ORC_P), sizeof(double) * n);
+
+        /* 1. correct_uvw (CPU/Inamuro.cpp:794-823): u -= (p_e - p_w)/(2 rho') */
+        memcpy(lp, la[5], sizeof(double) * n);
+        for (int x = 0; x < lx; ++x)
+            for (int y = 0; y < ly; ++y)
+                for (int z = 0; z < lz; ++z) {
+                    const size_t i = IX(s, x, y, z);
+                    const size_t ie = IX(s, (x == lx - 1) ? lx - 2 : x + 1, y, z);
+                    const size_t iw = IX(s, (x == 0) ? 1 : x - 1, y, z);
+                    const size_t in = IX(s, x, (y == ly - 1) ? 0 : y + 1, z);
+                    const size_t is = IX(s, x, (y == 0) ? ly - 1 : y - 1, z);
+                    const size_t it = IX(s, x, y, (z == lz - 1) ? 0 : z + 1);
+                    const size_t ib = IX(s, x, y, (z == 0) ? lz - 1 : z - 1);
+                    const double inv = 1.0 / (2.0 * rho_new[i]);
+                    lp[ie] -= la[2][i] * inv; lp[iw] += la[2][i] * inv;
+                    lp[in] -= la[3][i] * inv; lp[is] += la[3][i] * inv;
+                    lp[it] -= la[4][i] * inv; lp[ib] += la[4][i] * inv;
+                    la[1][i] += (la[2][i] * (pfin[ie] - pfin[iw]) + la[3][i] * (pfin[in] - pfin[is]) +
+                                 la[4][i] * (pfin[it] - pfin[ib])) /
+                                (2.0 * rho_new[i] * rho_new[i]);
+                }
+
+        /* 2. Poisson iterations, last first (CPU/Inamuro.cpp:673-716) */
+        memset(lh, 0, sizeof(double) * qn);
+        memset(ltau, 0, sizeof(double) * n);
+        memset(lS, 0, sizeof(double) * n);
+        double lam_target = 0.0;
+        for (uint64_t it = nit; it >= 1; --it) {
+            if (it % s->check_interval == 0) {
+                /* p -= shift, h -= E shift, shift = mean(p) - target */
+                double L = 0.0;
+                for (size_t i = 0; i < n; ++i) {
+                    double t = lp[i];
+                    for (int q = 0; q < Q; ++q) t += EW[q] * lh[(size_t)q * n + i];
+                    L += t;
+                }
+                for (size_t i = 0; i < n; ++i) lp[i] -= L / (double)n;
+                lam_target += L;
+            }
+            /* p = sum_q h_q */
+            for (int q = 0; q < Q; ++q)
+                for (size_t i = 0; i < n; ++i) lh[(size_t)q * n + i] += lp[i];
+            adj_stream_slip(s, lh);
+            /* h* = h (1 - 1/tau) + E p / tau - E S / 3 at the iterate it-1 */
+            const double* hk = rec_h + (it - 1) * qn;
+            const double* pk = rec_p + (it - 1) * n;
+            for (size_t i = 0; i < n; ++i) {
+                const double t = tau[i];
+                double np = 0.0, ns = 0.0, nt = 0.0;
+                for (int q = 0; q < Q; ++q) {
+                    double* l = &lh[(size_t)q * n + i];
+                    np += *l * EW[q] / t;
+                    ns -= *l * EW[q] / 3.0;
+                    nt += *l * (hk[(size_t)q * n + i] - EW[q] * pk[i]) / (t * t);
+                    *l = *l * (1.0 - 1.0 / t);
+                }
+                lp[i] = np;
+                lS[i] += ns;
+                ltau[i] += nt;
+            }
+        }
+        /* solve entry: h_0 = E p_start (0 on the first step), p_0 = p_start,
+         * target = mean(p_start); source projection S = div - mean(div) */
+        for (size_t i = 0; i < n; ++i) {
+            double b = lp[i] + lam_target / (double)n;
+            if (!c->first)
+                for (int q = 0; q < Q; ++q) b += EW[q] * lh[(size_t)q * n + i];
+            la[5][i] = b;
+        }
+        double mS = 0.0;
+        for (size_t i = 0; i < n; ++i) mS += lS[i];
+        mS /= (double)n;
+        for (int j = 0; j < GA_NUM; ++j) memset(ga[j], 0, sizeof(double) * n);
+        for (size_t i = 0; i < n; ++i) {
+            la[1][i] -= ltau[i] / (rho_new[i] * rho_new[i]);
+            ga[GA_UX][i] = ga[GA_VY][i] = ga[GA_WZ][i] = lS[i] - mS;
+        }
+
+        /* 3. getMacro (CPU/Inamuro.cpp:294-346) plus the streamed f, g the
+         *    next step read */
+        for (size_t i = 0; i < n; ++i) {
+            const double af = la[0][i] + la[1][i] * drho_dfei(P, fei_new[i]);
+            for (int q = 0; q < Q; ++q) {
+                lf[(size_t)q * n + i] = cf[(size_t)q * n + i] + af;
+                lg[(size_t)q * n + i] = cg[(size_t)q * n + i] + UC[q] * la[2][i] + VC[q] * la[3][i] + WC[q] * la[4][i];
+            }
+        }
+        /* 4. slip + stream of f and g */
+        adj_stream_slip(s, lf);
+        adj_stream_slip(s, lg);
+
+        /* 5. collision (CPU/Inamuro.cpp:134-250) at the step-start fields */
+        for (int j = 0; j < 5; ++j) memcpy(A(s, ORC_FEI + j), c->m[j], sizeof(double) * n);
+        gradients(s);
+        for (int j = 0; j < 5; ++j) memset(la[j], 0, sizeof(double) * n);
+        const double tmp1 = (P->mu_L - P->mu_G) / (P->rho_L - P->rho_G);
+        const double V = 1.5 * (P->taug - 0.5) * 2.0;
+        for (size_t i = 0; i < n; ++i) {
+            const double phi = c->m[0][i], rho = c->m[1][i];
+            const double u[3] = {c->m[2][i], c->m[3][i], c->m[4][i]};
+            const double gp[3] = {A(s, ORC_FEI_X)[i], A(s, ORC_FEI_Y)[i], A(s, ORC_FEI_Z)[i]};
+            const double gr[3] = {A(s, ORC_RHO_X)[i], A(s, ORC_RHO_Y)[i], A(s, ORC_RHO_Z)[i]};
+            const double Lu[3] = {A(s, ORC_U_LAP)[i], A(s, ORC_V_LAP)[i], A(s, ORC_W_LAP)[i]};
+            const double Lphi = A(s, ORC_FEI_LAP)[i];
+            const double sum_rho = gr[0] * gr[0] + gr[1] * gr[1] + gr[2] * gr[2];
+            const double mu = (rho - P->rho_G) * tmp1 + P->mu_G;
+            const double dp0 = P->T / ((1.0 - P->b * phi) * (1.0 - P->b * phi)) - 2.0 * P->a * phi;
+            double a_phi = 0.0, a_rho = 0.0, a_u[3] = {0, 0, 0}, a_gp[3] = {0, 0, 0}, a_gr[3] = {0, 0, 0};
+            double a_Lu[3] = {0, 0, 0}, a_D[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+            double a_p0 = 0.0, a_sf = 0.0, a_sr = 0.0, a_vel = 0.0;
+            for (int q = 0; q < Q; ++q) {
+                const double cq[3] = {UC[q], VC[q], WC[q]};
+                const size_t iq = (size_t)q * n + i;
+                double af = lf[iq], ag = lg[iq];
+                if (!c->first) {
+                    cf[iq] = af * (1.0 - 1.0 / P->tauf);
+                    cg[iq] = ag * (1.0 - 1.0 / P->taug);
+                    const double cl = cq[0] * Lu[0] + cq[1] * Lu[1] + cq[2] * Lu[2];
+                    for (int a = 0; a < 3; ++a) a_Lu[a] += ag * 3.0 * EW[q] * mu / rho * cq[a];
+                    a_rho += ag * 3.0 * EW[q] * cl * (tmp1 / rho - mu / (rho * rho));
+                    af /= P->tauf;
+                    ag /= P->taug;
+                } else {
+                    cf[iq] = cg[iq] = 0.0;
+                }
+                const double un = cq[0] * u[0] + cq[1] * u[1] + cq[2] * u[2];
+                const double fe = cq[0] * gp[0] + cq[1] * gp[1] + cq[2] * gp[2];
+                const double re = cq[0] * gr[0] + cq[1] * gr[1] + cq[2] * gr[2];
+                const double cs = cq[0] * cq[0] + cq[1] * cq[1] + cq[2] * cq[2];
+                const double Gr = 4.5 * re * re - 1.5 * sum_rho * cs;
+                /* feq = HW fei + FW (p0 - k_f fei lap - k_f |grad fei|^2/6)
+                 *       + 3 E fei (c.u) + E k_f (4.5 (c.grad fei)^2 - 1.5 |grad fei|^2 |c|^2) */
+                a_phi += af * (HW[q] + 3.0 * EW[q] * un);
+                a_p0 += af * FW[q];
+                a_sf += af * EW[q] * P->k_f * (-1.5 * cs);
+                for (int a = 0; a < 3; ++a) {
+                    a_gp[a] += af * EW[q] * P->k_f * 9.0 * fe * cq[a];
+                    a_u[a] += af * 3.0 * EW[q] * phi * cq[a];
+                }
+                /* geq = E velPart + E k_g/rho Grho - 2/3 FW k_g/rho |grad rho|^2 */
+                const double vb = ag * EW[q];
+                a_vel += vb;
+                for (int a = 0; a < 3; ++a) {
+                    a_u[a] += vb * (3.0 + 9.0 * un) * cq[a];
+                    for (int b = 0; b < 3; ++b) a_D[a * 3 + b] += vb * V * cq[a] * cq[b];
+                }
+                const double gb = vb * P->k_g / rho;
+                for (int a = 0; a < 3; ++a) a_gr[a] += gb * 9.0 * re * cq[a];
+                a_sr += gb * (-1.5 * cs) - ag * 2.0 / 3.0 * FW[q] * P->k_g / rho;
+                a_rho += -vb * P->k_g * Gr / (rho * rho) + ag * 2.0 / 3.0 * FW[q] * P->k_g * sum_rho / (rho * rho);
+                kg_bar += vb * Gr / rho - ag * 2.0 / 3.0 * FW[q] * sum_rho / rho;
+            }
+            for (int a = 0; a < 3; ++a) a_u[a] -= 3.0 * u[a] * a_vel;
+            a_phi += a_p0 * (dp0 - P->k_f * Lphi);
+            a_sf += a_p0 * (-P->k_f / 6.0);
+            for (int a = 0; a < 3; ++a) {
+                a_gp[a] += a_sf * 2.0 * gp[a];
+                a_gr[a] += a_sr * 2.0 * gr[a];
+            }
+            la[0][i] = a_phi;
+            la[1][i] = a_rho;
+            la[2][i] = a_u[0]; la[3][i] = a_u[1]; la[4][i] = a_u[2];
+            ga[GA_UX][i] += a_D[0]; ga[GA_UY][i] += a_D[1]; ga[GA_UZ][i] += a_D[2];
+            ga[GA_VX][i] += a_D[3]; ga[GA_VY][i] += a_D[4]; ga[GA_VZ][i] += a_D[5];
+            ga[GA_WX][i] += a_D[6]; ga[GA_WY][i] += a_D[7]; ga[GA_WZ][i] += a_D[8];
+            ga[GA_RX][i] = a_gr[0]; ga[GA_RY][i] = a_gr[1]; ga[GA_RZ][i] = a_gr[2];
+            ga[GA_FX][i] = a_gp[0]; ga[GA_FY][i] = a_gp[1]; ga[GA_FZ][i] = a_gp[2];
+            ga[GA_ULAP][i] = a_Lu[0]; ga[GA_VLAP][i] = a_Lu[1]; ga[GA_WLAP][i] = a_Lu[2];
+            ga[GA_FLAP][i] = a_p0 * (-P->k_f * phi);
+        }
+        /* 6. stencils (CPU/Inamuro.cpp:140-148) */
+        adj_stencil(s, ga[GA_UX], ga[GA_UY], ga[GA_UZ], ga[GA_ULAP], la[2]);
+        adj_stencil(s, ga[GA_VX], ga[GA_VY], ga[GA_VZ], ga[GA_VLAP], la[3]);
+        adj_stencil(s, ga[GA_WX], ga[GA_WY], ga[GA_WZ], ga[GA_WLAP], la[4]);
+        adj_stencil(s, ga[GA_RX], ga[GA_RY], ga[GA_RZ], NULL, la[1]);
+        adj_stencil(s, ga[GA_FX], ga[GA_FY], ga[GA_FZ], ga[GA_FLAP], la[0]);
+    }
+    /* initial state: fei = phi0, rho = rho_G + (phi0 - fei_min) * scale */
+    {
+        const double scale = (P->rho_L - P->rho_G) / (P->fei_max - P->fei_min);
+        for (size_t i = 0; i < n; ++i) la[0][i] += la[1][i] * scale;
+        store_canonical(s, ORC_P_PREV, la[0], grad_phi0);
+        *grad_kg = kg_bar;
+    }
+    rc = 0;
+done:
+    orc_set_schedule(s, NULL, 0);
+    for (int k = 0; ck && k < steps; ++k) {
+        for (int j = 0; j < 6; ++j) free(ck[k].m[j]);
+        free(ck[k].f); free(ck[k].g); free(ck[k].p_prev);
+    }
+    free(ck); free(tgt); free(rec_h); free(rec_p); free(fei_new); free(rho_new); free(tau); free(pfin);
+    free(lf); free(lg); free(cf); free(cg); free(lh); free(lp); free(ltau); free(lS);
+    for (int k = 0; k < 6; ++k) free(la[k]);
+    for (int k = 0; k < GA_NUM; ++k) free(ga[k]);
+    return rc;
+}

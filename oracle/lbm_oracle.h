@@ -126,6 +126,21 @@ int orc_is_first_step(const orc_sim* s);
 void orc_set_first_step(orc_sim* s, int first);
 void orc_poisson_diag_get(const orc_sim* s, orc_poisson_diag* out);
 
+/* Extension: the next `count` Poisson solves run exactly iters[i] iterations
+ * (gauging at every check) -- the frozen map that finite differences of the
+ * adjoint are taken on. */
+int orc_set_schedule(orc_sim* s, const uint64_t* iters, int count);
+
+/* Extension (no counterpart in the reference): interface-shape objective
+ * J = sum (phi_T - target)^2 after `steps` steps from phi0 (rho0 linear in
+ * phi0 as in place_droplet, u = v = w = p = 0, h = 0, first step) with Poisson
+ * iteration counts iters[0..steps-1], and its discrete adjoint: dJ/dphi0
+ * (canonical layout) and dJ/dk_g.  Both reset the simulation state. */
+int orc_objective(orc_sim* s, const double* phi0, const double* target, int steps,
+                  const uint64_t* iters, double* J);
+int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
+                const uint64_t* iters, double* J, double* grad_phi0, double* grad_kg);
+
 /* Closed-form helpers exposed for unit tests. */
 double orc_eos(const orc_params* p, double fei);                 /* CPU/Inamuro.cpp:843-847 */
 double orc_rho_of_phi(const orc_params* p, double fei);           /* CPU/Inamuro.cpp:313-327 */

@@ -129,6 +129,10 @@ def lib():
         L.orc_rho_of_phi.restype = ctypes.c_double
         L.orc_rho_of_phi.argtypes = [ctypes.POINTER(_CParams), ctypes.c_double]
         L.orc_poisson_iterations.argtypes = [vp, ctypes.c_int]
+        u64p = ctypes.POINTER(ctypes.c_uint64)
+        L.orc_set_schedule.argtypes = [vp, u64p, ctypes.c_int]
+        L.orc_objective.argtypes = [vp, dp, dp, ctypes.c_int, u64p, dp]
+        L.orc_adjoint.argtypes = [vp, dp, dp, ctypes.c_int, u64p, dp, dp, dp]
         _lib = L
     return _lib
 
@@ -230,6 +234,36 @@ class Oracle:
 
     def poisson_iterations(self, k: int) -> None:
         lib().orc_poisson_iterations(self.h, k)
+
+    def set_schedule(self, iters) -> None:
+        """Freeze the iteration counts of the next len(iters) Poisson solves."""
+        a = np.ascontiguousarray(np.asarray(iters, dtype=np.uint64))
+        self._check(lib().orc_set_schedule(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(a)))
+
+    def _adj_args(self, phi0, target, iters):
+        phi0 = np.ascontiguousarray(np.asarray(phi0, dtype=np.float64).reshape(self.shape))
+        target = np.ascontiguousarray(np.asarray(target, dtype=np.float64).reshape(self.shape))
+        it = np.ascontiguousarray(np.asarray(iters, dtype=np.uint64))
+        dptr = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+        return phi0, target, it, dptr
+
+    def objective(self, phi0, target, iters) -> float:
+        """J = sum (phi_T - target)^2 of the frozen-schedule map from phi0."""
+        phi0, target, it, dptr = self._adj_args(phi0, target, iters)
+        J = ctypes.c_double()
+        self._check(lib().orc_objective(self.h, dptr(phi0), dptr(target), len(it),
+                                        it.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), ctypes.byref(J)))
+        return J.value
+
+    def adjoint(self, phi0, target, iters):
+        """(J, dJ/dphi0, dJ/dk_g) of the frozen-schedule map."""
+        phi0, target, it, dptr = self._adj_args(phi0, target, iters)
+        J, gk = ctypes.c_double(), ctypes.c_double()
+        g = np.empty(self.shape, dtype=np.float64)
+        self._check(lib().orc_adjoint(self.h, dptr(phi0), dptr(target), len(it),
+                                      it.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)),
+                                      ctypes.byref(J), dptr(g), ctypes.byref(gk)))
+        return J.value, g, gk.value
 
 
 # ---------------------------------------------------------------------------
