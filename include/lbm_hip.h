@@ -103,6 +103,18 @@ typedef struct lbm_perf_s {
     uint64_t poisson_iter_launches;
 } lbm_perf_t;
 
+/* Cost record of the last lbm_adjoint call (no counterpart in the reference). */
+typedef struct lbm_adjoint_stats_s {
+    int32_t steps;                   /* T */
+    int32_t checkpoint_interval;     /* K */
+    int64_t store_slots;             /* Poisson iterates recorded per block */
+    uint64_t forward_iterations;     /* Poisson iterations of the forward pass */
+    uint64_t recomputed_iterations;  /* Poisson iterations re-run in the reverse sweep */
+    uint64_t adjoint_iterations;     /* adjoint Poisson iterations */
+    double forward_ms;               /* forward pass with checkpoint saves */
+    double reverse_ms;               /* reverse sweep, replays and recomputation included */
+} lbm_adjoint_stats_t;
+
 /* InamuroCUDA(const Inamuro&) (GPU/InamuroCUDA.hpp:46): allocates device state
  * for the grid in *params.  Fields start as Inamuro::initializeArrays leaves
  * them (CPU/Inamuro.cpp:511-548) until lbm_upload is called. */
@@ -194,6 +206,28 @@ int lbm_perf_reset(void* ctx);
 
 /* Device global sums in FP64: sum of phi (mass) and of |u| over the grid. */
 int lbm_mass(void* ctx, double* sum_phi);
+
+/* Discrete adjoint (no counterpart in the reference; FP64 contexts only).
+ * Objective J = sum (phi_T - target)^2 after `steps` steps from phi0 with
+ * rho0 = rho_G + (phi0 - fei_min)(rho_L - rho_G)/(fei_max - fei_min) (the
+ * density rule of place_droplet, GPU/Inamuro.cpp:521-527), u = v = w = p = 0
+ * and the first-step flag set (CPU/Inamuro.cpp:150-151).  Returns J, dJ/dphi0
+ * (canonical layout; NULL skips) and dJ/dk_g (NULL skips).  The Poisson
+ * iteration counts are `iterations[0..steps-1]` when given (positive multiples
+ * of the check interval), otherwise those the forward pass converges to; the
+ * gradient is that of the step with these counts frozen, transposing every
+ * executed iteration and gauge.  State is checkpointed every
+ * `checkpoint_interval` steps and recomputed in the reverse sweep.  On return
+ * the context holds the state at the end of the forward pass. */
+int lbm_adjoint(void* ctx, const double* phi0, const double* target, int32_t steps,
+                int32_t checkpoint_interval, const uint64_t* iterations, double* objective,
+                double* grad_phi0, double* grad_kg);
+
+/* Device memory for recorded Poisson iterates in the reverse sweep (0 =
+ * automatic, 90% of what is free beyond 2 GiB).  A solve longer than the
+ * recorded block is replayed in blocks from raw-h snapshots. */
+int lbm_set_adjoint_memory(void* ctx, int64_t iterate_bytes);
+int lbm_adjoint_stats(const void* ctx, lbm_adjoint_stats_t* out);
 
 #ifdef __cplusplus
 }

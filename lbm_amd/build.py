@@ -28,6 +28,7 @@ COMMON = [
 UNITS = [
     ("kernels_f64.hip", ["-ffp-contract=off"]),
     ("kernels_mixed.hip", ["-ffp-contract=fast"]),
+    ("kernels_adjoint.hip", ["-ffp-contract=off"]),
     ("capi.cpp", ["-x", "hip", "-ffp-contract=off"]),
 ]
 

@@ -17,6 +17,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -101,6 +102,8 @@ struct Ctx {
     std::vector<uint64_t> history;   // iterations of every step since the last upload
     std::vector<uint64_t> schedule;  // frozen per-step iteration counts (empty = converge)
     size_t sched_pos = 0;
+    int64_t adj_store_bytes = 0;     // device bytes for recorded Poisson iterates (0 = automatic)
+    lbm_adjoint_stats_t adj_stats{};
 
     bool timing = false;
     lbm_perf_t perf{};
@@ -585,6 +588,344 @@ void ensure_scratch(Ctx& c)
     if (!c.scratch) c.scratch = dalloc(lbm::Q * c.n());
 }
 
+
+// ---- discrete adjoint (no counterpart in the reference) ---------------------
+//
+// Checkpoint-and-recompute reverse sweep of the FP64 step with frozen Poisson
+// iteration counts:
+//   forward pass      state saved every K steps (37 doubles per cell)
+//   per segment       replay from its checkpoint, keeping each step's start
+//   per step, reverse recompute collide + macro + the Poisson solve, recording
+//                     its iterates (L slots; longer solves are split into
+//                     blocks of L iterations restarted from raw-h snapshots),
+//                     then the adjoint kernels of kernels_adjoint.hip.
+// The recorded iterate of iteration k is the raw streamed h it writes, so
+// recording costs no traffic beyond the forward kernel's own.
+
+constexpr size_t STATE_PER_CELL = 7 + 2 * lbm::Q;  // fei rho u v w p p_prev, f, g
+
+struct StepState {
+    double* buf = nullptr;
+    FG fg = FG::First;
+    HState hs = HState::Equilibrium;
+};
+
+struct AdjWork {
+    std::vector<StepState> ckpt, seg;
+    StepState final_state;
+    double* slots = nullptr;  // L raw-h slots, 15n each
+    std::vector<double*> snaps;
+    double* pstore = nullptr;  // gauged p of every check, n each
+    double* shifts = nullptr;  // gauge shift of every check
+    double *lfei = nullptr, *lrho = nullptr, *lu = nullptr, *lv = nullptr, *lw = nullptr;
+    double *lp = nullptr, *lps = nullptr, *lpf = nullptr, *lS = nullptr, *ltau = nullptr;
+    double *lmac = nullptr, *pc = nullptr, *parts = nullptr, *target = nullptr;
+    double* L[2] = {nullptr, nullptr};
+    double* cf[2] = {nullptr, nullptr};
+    double* cg[2] = {nullptr, nullptr};
+    lbm::AdjState* as = nullptr;
+    lbm::AdjState* has = nullptr;
+    uint64_t L_slots = 0;
+
+    ~AdjWork()
+    {
+        auto F = [](auto*& p) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+        };
+        for (auto& s : ckpt) F(s.buf);
+        for (auto& s : seg) F(s.buf);
+        F(final_state.buf);
+        F(slots);
+        for (auto& p : snaps) F(p);
+        F(pstore); F(shifts);
+        for (double** p : {&lfei, &lrho, &lu, &lv, &lw, &lp, &lps, &lpf, &lS, &ltau, &lmac, &pc, &parts, &target})
+            F(*p);
+        for (int i = 0; i < 2; ++i) { F(L[i]); F(cf[i]); F(cg[i]); }
+        F(as);
+        if (has) (void)hipHostFree(has);
+    }
+    double* slot(const Ctx& c, uint64_t m) const { return slots + (m - 1) * lbm::Q * c.n(); }
+};
+
+void d2d(Ctx& c, void* dst, const void* src, size_t bytes)
+{
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, c.stream));
+}
+
+void alloc_state(Ctx& c, StepState& st) { st.buf = dalloc(STATE_PER_CELL * c.n()); }
+
+void save_state(Ctx& c, StepState& st)
+{
+    const size_t n = c.n(), nb = n * sizeof(double);
+    const void* src[7] = {c.fei, c.rho, c.u, c.v, c.w, c.p[c.pc], c.p_prev};
+    for (int i = 0; i < 7; ++i) d2d(c, st.buf + i * n, src[i], nb);
+    d2d(c, st.buf + 7 * n, c.f[c.fcur], lbm::Q * nb);
+    d2d(c, st.buf + (7 + lbm::Q) * n, c.gd[c.fcur], lbm::Q * nb);
+    st.fg = c.fg;
+    st.hs = c.hs;
+}
+
+void restore_state(Ctx& c, const StepState& st)
+{
+    const size_t n = c.n(), nb = n * sizeof(double);
+    void* dst[7] = {c.fei, c.rho, c.u, c.v, c.w, c.p[c.pc], c.p_prev};
+    for (int i = 0; i < 7; ++i) d2d(c, dst[i], st.buf + i * n, nb);
+    d2d(c, c.f[c.fcur], st.buf + 7 * n, lbm::Q * nb);
+    d2d(c, c.gd[c.fcur], st.buf + (7 + lbm::Q) * n, lbm::Q * nb);
+    c.fg = st.fg;
+    c.hs = st.hs;
+}
+
+// Recompute Poisson iterations a0+1..a1 of the current step (collide + macro
+// already run).  Block mode writes iteration k's raw h into slot k-a0; the
+// snapshot pass (a0 = 0, a1 = nit) ping-pongs through slots 1 and 2 and keeps
+// the raw h of iterations L, 2L, ... in the snapshot buffers.  Every check
+// stores its gauged p and shift.
+void adj_forward(Ctx& c, AdjWork& a, const double* p_start, uint64_t a0, uint64_t a1, bool snapshot_pass)
+{
+    const uint64_t ci = static_cast<uint64_t>(c.check_interval);
+    const size_t n = c.n();
+    int mode = lbm::H_EQUILIBRIUM;
+    const double* in = a.slot(c, 1);  // not read in the equilibrium mode
+    if (a0 > 0) {
+        mode = lbm::H_GAUGED;
+        in = a.snaps[a0 / a.L_slots - 1];
+        d2d(c, &c.dst->shift, a.shifts + a0 / ci, sizeof(double));
+    }
+    for (uint64_t k = a0 + 1; k <= a1; ++k) {
+        double* out;
+        if (!snapshot_pass) out = a.slot(c, k - a0);
+        else if (k % a.L_slots == 0 && k < a1) out = a.snaps[k / a.L_slots - 1];
+        else out = a.slot(c, 1 + (k & 1));
+        const double* pbuf = mode == lbm::H_GAUGED ? a.pstore + ((k - 1) / ci) * n : p_start;
+        lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, pbuf, D<double>(c.tau), D<double>(c.src), c.dst);
+        if (k % ci == 0) {
+            lbm::f64::gauge(c.stream, c.g, c.ph, out, a.pstore + (k / ci) * n, c.p_prev, c.partials, c.dst);
+            d2d(c, a.shifts + k / ci, &c.dst->shift, sizeof(double));
+            mode = lbm::H_GAUGED;
+        } else {
+            mode = lbm::H_STREAMED;
+        }
+        in = out;
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// Reverse of one step whose start state is st (already restored into c).
+void adj_step(Ctx& c, AdjWork& a, const StepState& st, uint64_t nit, int ccur, lbm_adjoint_stats_t& stats)
+{
+    hipStream_t s = c.stream;
+    const Grid& g = c.g;
+    const size_t n = c.n();
+    const uint64_t ci = static_cast<uint64_t>(c.check_interval);
+    const uint64_t L = a.L_slots;
+    const double* s_fei = st.buf;
+    const double* s_rho = st.buf + n;
+    const double* s_u = st.buf + 2 * n;
+    const double* s_v = st.buf + 3 * n;
+    const double* s_w = st.buf + 4 * n;
+    const double* p_start = st.buf + 5 * n;
+    const bool first = st.fg == FG::First;
+
+    step_fg(c);  // fei', rho', u', tau_h, S
+    const uint64_t nblocks = (nit + L - 1) / L;
+    if (nblocks == 1) adj_forward(c, a, p_start, 0, nit, false);
+    else adj_forward(c, a, p_start, 0, nit, true);
+    stats.recomputed_iterations += nit;
+    const double* p_final = a.pstore + (nit / ci) * n;
+
+    lbm::adj::correct(s, g, c.ph, p_final, D<double>(c.rho), a.lu, a.lv, a.lw, a.lp, a.lrho, a.lpf, a.parts, a.as);
+    HIPCHK(hipMemsetAsync(a.lS, 0, n * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(a.ltau, 0, n * sizeof(double), s));
+    int lcur = 0;
+    for (uint64_t j = nblocks; j-- > 0;) {
+        const uint64_t a0 = j * L, a1 = std::min(nit, a0 + L);
+        if (nblocks > 1) {
+            adj_forward(c, a, p_start, a0, a1, false);
+            stats.recomputed_iterations += a1 - a0;
+        }
+        for (uint64_t k = a1; k > a0; --k) {
+            const int src = k == nit ? lbm::ADJ_SRC_INIT : (k % ci == 0 ? lbm::ADJ_SRC_LAZY : lbm::ADJ_SRC_PLAIN);
+            const int mode = k == 1 ? lbm::H_EQUILIBRIUM
+                                    : ((k - 1) % ci == 0 ? lbm::H_GAUGED : lbm::H_STREAMED);
+            const double* raw = nullptr;
+            if (k >= a0 + 2) raw = a.slot(c, k - 1 - a0);
+            else if (a0 > 0) raw = a.snaps[a0 / L - 1];
+            const double* pin = mode == lbm::H_EQUILIBRIUM ? p_start
+                               : (mode == lbm::H_GAUGED ? a.pstore + ((k - 1) / ci) * n : nullptr);
+            const double* shift = mode == lbm::H_GAUGED ? a.shifts + (k - 1) / ci : nullptr;
+            double* out = k == 1 ? a.lps : a.L[1 - lcur];
+            lbm::adj::poisson(s, g, c.ph, src, mode, a.L[lcur], a.lpf, a.as, raw, pin, shift,
+                              D<double>(c.tau), a.lS, a.ltau, out, a.parts);
+            if (k > 1) lcur = 1 - lcur;
+        }
+        HIPCHK(hipGetLastError());
+    }
+    stats.adjoint_iterations += nit;
+    lbm::adj::source(s, g, c.ph, D<double>(c.fei), D<double>(c.rho), a.ltau, a.lS, a.lfei, a.as, a.lrho,
+                     a.lps, a.lmac, a.parts);
+    lbm::adj::collide(s, first ? 1 : 0, g, c.ph, s_fei, s_rho, s_u, s_v, s_w, a.cf[ccur], a.cg[ccur], a.lmac,
+                      a.lu, a.lv, a.lw, a.lS, a.as, a.cf[1 - ccur], a.cg[1 - ccur], a.pc, a.parts);
+    lbm::adj::stencil(s, g, a.pc, a.lfei, a.lrho, a.lu, a.lv, a.lw);
+    HIPCHK(hipGetLastError());
+    std::swap(a.lp, a.lps);
+}
+
+// Largest block length L (a multiple of the check interval, at most the
+// longest solve) whose slots and snapshots fit in `slots_fit` buffers of 15n.
+uint64_t choose_block(uint64_t max_nit, uint64_t ci, uint64_t slots_fit)
+{
+    for (uint64_t L = (std::min(max_nit, slots_fit) / ci) * ci; L >= ci; L -= ci) {
+        const uint64_t snaps = (max_nit + L - 1) / L - 1;
+        if (std::max<uint64_t>(L, 2) + snaps <= slots_fit) return L;
+    }
+    return 0;
+}
+
+void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K, const uint64_t* iters,
+                 double* J, double* grad_phi0, double* grad_kg)
+{
+    if (c.mixed) throw ApiError{LBM_ERR_ARG, "the adjoint runs on an FP64 context"};
+    if (!phi0 || !target || !J || T < 1 || K < 1)
+        throw ApiError{LBM_ERR_ARG, "adjoint needs phi0, target, objective, steps >= 1 and checkpoint interval >= 1"};
+    const uint64_t ci = static_cast<uint64_t>(c.check_interval);
+    if (iters)
+        for (int k = 0; k < T; ++k)
+            if (iters[k] == 0 || iters[k] % ci != 0)
+                throw ApiError{LBM_ERR_ARG, "scheduled iteration counts must be positive multiples of the check interval"};
+    hipStream_t s = c.stream;
+    const Grid& g = c.g;
+    const size_t n = c.n();
+    const lbm_params_t& P = c.P;
+    lbm_adjoint_stats_t stats{};
+    stats.steps = T;
+    stats.checkpoint_interval = K;
+    AdjWork a;
+
+    // Initial state: rho0 linear in phi0 (place_droplet's rule), rest zero.
+    HostFields h0;
+    h0.fei.assign(phi0, phi0 + n);
+    h0.rho.resize(n);
+    for (size_t i = 0; i < n; ++i)
+        h0.rho[i] = P.rho_G + (phi0[i] - P.fei_min) * (P.rho_L - P.rho_G) / (P.fei_max - P.fei_min);
+    h0.u.assign(n, 0.0);
+    h0.v.assign(n, 0.0);
+    h0.w.assign(n, 0.0);
+    h0.p.assign(n, 0.0);
+    upload_host(c, h0);
+    c.hs = HState::Equilibrium;  // h = E p = 0: the h = 0 of a fresh solver
+    if (iters) c.schedule.assign(iters, iters + T);
+    else c.schedule.clear();
+    c.sched_pos = 0;
+
+    const int nseg = (T + K - 1) / K;
+    a.ckpt.resize(static_cast<size_t>(nseg));
+    for (auto& st : a.ckpt) alloc_state(c, st);
+    alloc_state(c, a.final_state);
+    sync(c);
+    auto t0 = std::chrono::steady_clock::now();
+    for (int k = 0; k < T; ++k) {
+        if (k % K == 0) save_state(c, a.ckpt[static_cast<size_t>(k / K)]);
+        step_one(c);
+    }
+    save_state(c, a.final_state);
+    sync(c);
+    auto t1 = std::chrono::steady_clock::now();
+    stats.forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    const std::vector<uint64_t> sched(c.history.end() - T, c.history.end());
+    const std::vector<uint64_t> history = c.history;
+    uint64_t max_nit = 0;
+    for (uint64_t v : sched) {
+        max_nit = std::max(max_nit, v);
+        stats.forward_iterations += v;
+    }
+
+    // Reverse-sweep buffers.
+    a.seg.resize(static_cast<size_t>(std::min(K, T)));
+    for (auto& st : a.seg) alloc_state(c, st);
+    for (double** p : {&a.lfei, &a.lrho, &a.lu, &a.lv, &a.lw, &a.lp, &a.lps, &a.lpf, &a.lS, &a.ltau, &a.lmac, &a.target})
+        *p = dalloc(n);
+    for (int i = 0; i < 2; ++i) {
+        a.L[i] = dalloc(lbm::Q * n);
+        a.cf[i] = dalloc(lbm::Q * n);
+        a.cg[i] = dalloc(lbm::Q * n);
+    }
+    a.pc = dalloc(lbm::adj::PC_SLOTS * n);
+    a.parts = dalloc(static_cast<size_t>(lbm::grid_blocks(g.n)));
+    a.pstore = dalloc((max_nit / ci + 1) * n);
+    a.shifts = dalloc(max_nit / ci + 1);
+    a.as = static_cast<lbm::AdjState*>(dalloc_bytes(sizeof(lbm::AdjState)));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&a.has), sizeof(lbm::AdjState), hipHostMallocDefault));
+    const size_t slot_bytes = lbm::Q * n * sizeof(double);
+    size_t budget = static_cast<size_t>(std::max<int64_t>(c.adj_store_bytes, 0));
+    if (budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        const size_t reserve = size_t(2) << 30;
+        budget = free_b > reserve ? (free_b - reserve) / 10 * 9 : 0;
+    }
+    const uint64_t L = choose_block(max_nit, ci, budget / slot_bytes);
+    if (L == 0) throw ApiError{LBM_ERR_NOMEM, "adjoint needs device memory for check_interval recorded Poisson iterates"};
+    a.L_slots = L;
+    stats.store_slots = static_cast<int64_t>(L);
+    a.slots = static_cast<double*>(dalloc_bytes(std::max<uint64_t>(L, 2) * slot_bytes));
+    a.snaps.resize((max_nit + L - 1) / L - 1, nullptr);
+    for (auto& p : a.snaps) p = dalloc(lbm::Q * n);
+
+    // Seed: J and dJ/dfei_T; every other adjoint starts at zero.
+    HIPCHK(hipMemcpyAsync(a.target, target, n * sizeof(double), hipMemcpyHostToDevice, s));
+    lbm::adj::seed(s, g, D<double>(c.fei), a.target, a.lfei, a.parts, a.as);
+    for (double* p : {a.lrho, a.lu, a.lv, a.lw, a.lp}) HIPCHK(hipMemsetAsync(p, 0, n * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(a.cf[0], 0, slot_bytes, s));
+    HIPCHK(hipMemsetAsync(a.cg[0], 0, slot_bytes, s));
+    HIPCHK(hipGetLastError());
+
+    const bool timing = c.timing;
+    const lbm_perf_t perf = c.perf;  // replays are not steps of the simulation
+    c.timing = false;
+    c.schedule = sched;
+    int ccur = 0;
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+        const int k0 = sg * K, k1 = std::min(T, k0 + K);
+        restore_state(c, a.ckpt[static_cast<size_t>(sg)]);
+        c.sched_pos = static_cast<size_t>(k0);
+        for (int k = k0; k < k1; ++k) {
+            save_state(c, a.seg[static_cast<size_t>(k - k0)]);
+            if (k + 1 < k1) step_one(c);
+        }
+        for (int k = k1 - 1; k >= k0; --k) {
+            const StepState& st = a.seg[static_cast<size_t>(k - k0)];
+            restore_state(c, st);
+            adj_step(c, a, st, sched[static_cast<size_t>(k)], ccur, stats);
+            ccur = 1 - ccur;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(a.has, a.as, sizeof(lbm::AdjState), hipMemcpyDeviceToHost, s));
+    std::vector<double> lf(n), lr(n);
+    HIPCHK(hipMemcpyAsync(lf.data(), a.lfei, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(lr.data(), a.lrho, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    sync(c);
+    auto t2 = std::chrono::steady_clock::now();
+    stats.reverse_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    // phi0 enters as fei and through rho0.
+    const double scale = (P.rho_L - P.rho_G) / (P.fei_max - P.fei_min);
+    if (grad_phi0)
+        for (size_t i = 0; i < n; ++i) grad_phi0[i] = lf[i] + lr[i] * scale;
+    *J = a.has->J;
+    if (grad_kg) *grad_kg = a.has->kg;
+
+    // Leave the context at the end of the forward pass.
+    restore_state(c, a.final_state);
+    sync(c);
+    c.timing = timing;
+    c.perf = perf;
+    c.schedule.clear();
+    c.sched_pos = 0;
+    c.history = history;
+    c.adj_stats = stats;
+}
+
 } // namespace
 
 extern "C" {
@@ -883,6 +1224,30 @@ int lbm_perf_get(const void* ctx, lbm_perf_t* out)
 int lbm_perf_reset(void* ctx)
 {
     return guarded(ctx, [&](Ctx& c) { c.perf = lbm_perf_t{}; });
+}
+
+int lbm_set_adjoint_memory(void* ctx, int64_t iterate_bytes)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        if (iterate_bytes < 0) throw ApiError{LBM_ERR_ARG, "iterate memory must be non-negative"};
+        c.adj_store_bytes = iterate_bytes;
+    });
+}
+
+int lbm_adjoint(void* ctx, const double* phi0, const double* target, int32_t steps,
+                int32_t checkpoint_interval, const uint64_t* iterations, double* objective,
+                double* grad_phi0, double* grad_kg)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        run_adjoint(c, phi0, target, steps, checkpoint_interval, iterations, objective, grad_phi0, grad_kg);
+    });
+}
+
+int lbm_adjoint_stats(const void* ctx, lbm_adjoint_stats_t* out)
+{
+    if (!ctx || !out) return LBM_ERR_ARG;
+    *out = static_cast<const Ctx*>(ctx)->adj_stats;
+    return LBM_OK;
 }
 
 int lbm_mass(void* ctx, double* sum_phi)

@@ -97,8 +97,48 @@ void correct(hipStream_t s, const Grid& g, const double* p, const double* rho, d
 void materialize(hipStream_t s, const Grid& g, const double* in, double* out);
 void equilibrium_h(hipStream_t s, const Grid& g, const double* p, double* h);
 void sum_field(hipStream_t s, const Grid& g, const double* a, double* partials, DevState* st);
+// getp + gauge of a check without the convergence test (recomputation of an
+// already converged solve): p = sum_q view(h), shift, p -= shift.
+void gauge(hipStream_t s, const Grid& g, const Phys& ph, const double* hin, double* p,
+           double* p_prev, double* partials, DevState* st);
 
 } // namespace f64
+
+// Discrete adjoint of the FP64 step (no counterpart in the reference).
+struct AdjState {
+    double J;           // objective of the forward pass
+    double kg;          // dJ/dk_g accumulated over the reverse sweep
+    double lam_shift;   // pending gauge adjoint Lambda/N, applied on read
+    double lam_target;  // adjoint of the gauge target mean(p_start), this step
+    double lS_mean;     // mean of the source adjoint (projection S - mean)
+    double pad;
+};
+enum : int { ADJ_SRC_PLAIN = 0, ADJ_SRC_LAZY = 1, ADJ_SRC_INIT = 2 };
+
+namespace adj {
+
+void seed(hipStream_t s, const Grid& g, const double* fei, const double* target, double* lfei,
+          double* parts, AdjState* as);
+void correct(hipStream_t s, const Grid& g, const Phys& ph, const double* p, const double* rho,
+             const double* lu, const double* lv, const double* lw, const double* lp, double* lrho,
+             double* lpf, double* parts, AdjState* as);
+void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, const double* Lin,
+             const double* lpf, AdjState* as, const double* raw, const double* pin,
+             const double* shift, const double* tau, double* lS, double* ltau, double* Lout,
+             double* parts);
+void source(hipStream_t s, const Grid& g, const Phys& ph, const double* fei, const double* rho,
+            const double* ltau, const double* lS, const double* lfei, AdjState* as, double* lrho,
+            double* lps, double* lmac, double* parts);
+void collide(hipStream_t s, int first, const Grid& g, const Phys& ph, const double* fei,
+             const double* rho, const double* u, const double* v, const double* w, const double* cf,
+             const double* cg, const double* lmac, const double* lu, const double* lv,
+             const double* lw, const double* lS, AdjState* as, double* cf_o, double* cg_o,
+             double* pc, double* parts);
+void stencil(hipStream_t s, const Grid& g, const double* pc, double* lfei, double* lrho, double* lu,
+             double* lv, double* lw);
+constexpr int PC_SLOTS = 24;  // per-cell adjoint slots of collide -> stencil
+
+} // namespace adj
 
 // Mixed precision: FP32 storage and collision arithmetic for f, g, h (as
 // deviations) and the macro fields; FP64 pressure, p_prev and every global sum.

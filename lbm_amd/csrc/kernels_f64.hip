@@ -706,6 +706,17 @@ void check(hipStream_t s, const Grid& g, const Phys& ph, double* p, double* p_pr
     k_check_finalize<<<1, BLOCK, 0, s>>>(parts_err, parts_fp, nb, dual, tol, st);
 }
 
+void gauge(hipStream_t s, const Grid& g, const Phys& ph, const double* hin, double* p,
+           double* p_prev, double* partials, DevState* st)
+{
+    const int nb = red_blocks(g.n);
+    double* parts_sum = partials;
+    double* parts_err = partials + RED_BLOCKS;
+    k_getp_sum<<<nb, BLOCK, 0, s>>>(g, hin, p, parts_sum);
+    k_gauge_finalize<<<1, BLOCK, 0, s>>>(ph, parts_sum, nb, st);
+    k_gauge_error<<<nb, BLOCK, 0, s>>>(g, p, p_prev, st, parts_err);
+}
+
 void gauge_finalize(hipStream_t s, const Phys& ph, const double* parts, int nparts, DevState* st)
 {
     k_gauge_finalize<<<1, BLOCK, 0, s>>>(ph, parts, nparts, st);

@@ -42,6 +42,7 @@ EXPORTED = (
     "lbm_poisson_diag", "lbm_synchronize", "lbm_set_timing", "lbm_perf_get",
     "lbm_perf_reset", "lbm_mass", "lbm_init_reference", "lbm_place_droplet",
     "lbm_reset_fields", "lbm_set_iteration_schedule", "lbm_iteration_history",
+    "lbm_adjoint", "lbm_set_adjoint_memory", "lbm_adjoint_stats",
 )
 
 
@@ -83,6 +84,16 @@ class PerformanceMetrics(ctypes.Structure):
     ]
 
 
+class AdjointStats(ctypes.Structure):
+    """lbm_adjoint_stats_t: cost record of the last adjoint call."""
+    _fields_ = [
+        ("steps", ctypes.c_int32), ("checkpoint_interval", ctypes.c_int32),
+        ("store_slots", ctypes.c_int64), ("forward_iterations", ctypes.c_uint64),
+        ("recomputed_iterations", ctypes.c_uint64), ("adjoint_iterations", ctypes.c_uint64),
+        ("forward_ms", ctypes.c_double), ("reverse_ms", ctypes.c_double),
+    ]
+
+
 _lib = None
 
 
@@ -117,6 +128,10 @@ def load_library():
     L.lbm_set_iteration_schedule.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64]
     L.lbm_iteration_history.argtypes = [vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_int64,
                                         ctypes.POINTER(ctypes.c_int64)]
+    L.lbm_adjoint.argtypes = [vp, dp, dp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64),
+                              dp, dp, dp]
+    L.lbm_set_adjoint_memory.argtypes = [vp, ctypes.c_int64]
+    L.lbm_adjoint_stats.argtypes = [vp, ctypes.POINTER(AdjointStats)]
     _lib = L
     return L
 
@@ -322,6 +337,48 @@ class InamuroHIP:
 
     def reset_perf(self):
         self._check(load_library().lbm_perf_reset(self._h))
+
+    # -- discrete adjoint (no counterpart in the reference) ----------------------
+    def initial_rho(self, phi0):
+        """rho0 of the adjoint's initial state: linear in phi0 (place_droplet's rule)."""
+        c = self._cp
+        return c.rho_G + (np.asarray(phi0, dtype=np.float64) - c.fei_min) * (c.rho_L - c.rho_G) / (c.fei_max - c.fei_min)
+
+    def adjoint(self, phi0, target, steps: int, checkpoint_interval: int = 100, iterations=None):
+        """(J, dJ/dphi0, dJ/dk_g) of J = sum (phi_T - target)^2 after `steps`
+        steps from phi0; `iterations` freezes the Poisson counts, otherwise
+        the forward pass's own counts are frozen for the reverse sweep."""
+        phi0 = self._field(phi0)
+        target = self._field(target)
+        grad = np.empty(self.shape)
+        J, gk = ctypes.c_double(), ctypes.c_double()
+        it_ptr = None
+        if iterations is not None:
+            its = np.ascontiguousarray(np.asarray(list(iterations), dtype=np.uint64))
+            if len(its) != steps:
+                raise ValueError("one iteration count per step")
+            it_ptr = its.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
+        self._check(load_library().lbm_adjoint(self._h, _ptr(phi0), _ptr(target), int(steps),
+                                               int(checkpoint_interval), it_ptr, ctypes.byref(J),
+                                               _ptr(grad), ctypes.byref(gk)))
+        return J.value, grad, gk.value
+
+    def objective(self, phi0, target, iterations) -> float:
+        """Forward pass of the adjoint's map with frozen iteration counts."""
+        phi0 = self._field(phi0)
+        self.initFromCPU(fei=phi0, rho=self.initial_rho(phi0), u=0.0, v=0.0, w=0.0, p=0.0)
+        self.set_iteration_schedule(iterations)
+        self.performTimeStepGPU(len(iterations))
+        fei = self.downloadFieldsToCPU()["fei"]
+        return float(((fei - self._field(target)) ** 2).sum())
+
+    def set_adjoint_memory(self, iterate_bytes: int):
+        self._check(load_library().lbm_set_adjoint_memory(self._h, int(iterate_bytes)))
+
+    def adjoint_stats(self) -> AdjointStats:
+        st = AdjointStats()
+        self._check(load_library().lbm_adjoint_stats(self._h, ctypes.byref(st)))
+        return st
 
     def mass(self) -> float:
         out = ctypes.c_double()

@@ -1,0 +1,143 @@
+"""HIP discrete adjoint (lbm_adjoint) against the oracle's adjoint and against
+central finite differences of the GPU's own frozen-schedule forward map.
+
+Bars: J and the gradients agree with the oracle to 1e-9 relative (both are
+the exact transpose of the same FP64 computation; they differ only in the
+association order of global sums); the checkpoint interval and the recorded
+iterate budget change how much is recomputed, never the result (bitwise);
+the directional derivative matches central differences (h = 1e-6 in phi0,
+1e-6 in k_g) to 1e-4 relative.
+"""
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from lbm_amd import hip
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+ADJ_TOL = 1e-9
+FD_TOL = 1e-4
+
+
+def params(lx=12, ly=10, lz=10, rho_L=10.0, DD=6.0):
+    return po.Params(lx=lx, ly=ly, lz=lz, rho_L=rho_L, rho_G=1.0, mu_L=0.008, mu_G=8e-5, tauf=1.0,
+                     taug=1.0, k_f=0.5, k_g=1e-4, T=0.035, a=1.0, b=6.7, fei_max=0.093, fei_min=0.016,
+                     fei_L=0.093, fei_G=0.016, DD=DD)
+
+
+def case(P, cx=None, seed=0):
+    o = po.Oracle(P)
+    o.place_droplet(P.lx / 2 if cx is None else cx, P.ly / 2, P.lz / 2, profile="tanh", width=1.5)
+    phi0 = o.get("fei")
+    rng = np.random.default_rng(seed)
+    target = phi0 + 0.002 * rng.standard_normal(phi0.shape)
+    return phi0, target, rng
+
+
+def rel(a, b):
+    return float(np.max(np.abs(a - b))) / max(float(np.max(np.abs(b))), 1e-300)
+
+
+def announce(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.mark.parametrize("rho_L,ci,steps,cx,lx", [(10.0, 10, 3, None, 12), (10.0, 10, 4, 1.5, 12),
+                                                (1000.0, 100, 3, None, 10)])
+def test_adjoint_matches_oracle(rho_L, ci, steps, cx, lx):
+    P = params(lx=lx, ly=8 if lx == 10 else 10, lz=8 if lx == 10 else 10, rho_L=rho_L)
+    announce(f"adjoint vs oracle: rho_L={rho_L} check={ci} steps={steps} cx={cx}")
+    phi0, target, _ = case(P, cx)
+    sim = hip.InamuroHIP(P)
+    sim.setPoisson(check_interval=ci)
+    J, g, gk = sim.adjoint(phi0, target, steps, checkpoint_interval=2)
+    iters = sim.iteration_history()[-steps:]
+    o = po.Oracle(P)
+    o.set_poisson(check_interval=ci)
+    Jo, go, gko = o.adjoint(phi0, target, iters)
+    assert abs(J - Jo) <= 1e-12 * Jo
+    assert rel(g, go) <= ADJ_TOL, rel(g, go)
+    assert abs(gk - gko) <= ADJ_TOL * abs(gko), (gk, gko)
+    st = sim.adjoint_stats()
+    assert st.steps == steps and st.forward_iterations == sum(iters)
+    assert st.adjoint_iterations == sum(iters)
+
+
+def test_recomputation_does_not_change_the_result():
+    """Checkpoint every step / every 2 / once, and a block of only one check
+    interval of recorded iterates (snapshot replays): all bitwise equal."""
+    P = params()
+    announce("adjoint recomputation invariance")
+    phi0, target, _ = case(P)
+    sim = hip.InamuroHIP(P)
+    sim.setPoisson(check_interval=10)
+    J0, g0, k0 = sim.adjoint(phi0, target, 4, checkpoint_interval=4)
+    iters = sim.iteration_history()[-4:]
+    assert max(iters) > 10
+    slot = 15 * P.lx * P.ly * P.lz * 8
+    m = max(iters)
+    for K, L in ((1, 0), (2, 0), (4, 10), (3, 20)):
+        # room for L recorded iterates plus the raw-h snapshots of the blocks
+        mem = (L + -(-m // L) - 1) * slot if L else 0
+        sim.set_adjoint_memory(mem)
+        J, g, k = sim.adjoint(phi0, target, 4, checkpoint_interval=K)
+        st = sim.adjoint_stats()
+        if L:
+            assert L <= st.store_slots < m and st.recomputed_iterations > sum(iters), (st.store_slots, m)
+        assert J == J0 and k == k0 and np.array_equal(g, g0), (K, L)
+
+
+def test_context_state_after_adjoint_is_the_forward_end():
+    P = params()
+    phi0, target, _ = case(P)
+    sim = hip.InamuroHIP(P)
+    sim.setPoisson(check_interval=10)
+    sim.adjoint(phi0, target, 3, checkpoint_interval=1)
+    after = sim.downloadFieldsToCPU()
+    fwd = hip.InamuroHIP(P)
+    fwd.setPoisson(check_interval=10)
+    fwd.initFromCPU(fei=phi0, rho=fwd.initial_rho(phi0), u=0.0, v=0.0, w=0.0, p=0.0)
+    fwd.step(3)
+    ref = fwd.downloadFieldsToCPU()
+    for k in ("fei", "rho", "u", "v", "w", "p"):
+        assert np.array_equal(after[k], ref[k]), k
+    assert sim.iteration_history() == fwd.iteration_history()
+
+
+def test_gradient_matches_central_fd_16_directions():
+    """Density ratio 1000, the reference's check interval; 16 seeded unit
+    Gaussian directions over phi0 (h = 1e-6), and k_g."""
+    P = params(lx=16, ly=12, lz=12, rho_L=1000.0, DD=8.0)
+    announce("adjoint vs central FD, 16 directions")
+    phi0, target, rng = case(P, seed=3)
+    sim = hip.InamuroHIP(P)
+    J, g, gk = sim.adjoint(phi0, target, 3, checkpoint_interval=2)
+    iters = sim.iteration_history()[-3:]
+    h = 1e-6
+    worst = 0.0
+    for _ in range(16):
+        d = rng.standard_normal(phi0.shape)
+        fd = (sim.objective(phi0 + h * d, target, iters) - sim.objective(phi0 - h * d, target, iters)) / (2 * h)
+        ad = float((g * d).sum())
+        worst = max(worst, abs(fd - ad) / abs(fd))
+    assert worst <= FD_TOL, worst
+    sp = hip.InamuroHIP(dataclasses.replace(P, k_g=P.k_g + h))
+    sm = hip.InamuroHIP(dataclasses.replace(P, k_g=P.k_g - h))
+    fd = (sp.objective(phi0, target, iters) - sm.objective(phi0, target, iters)) / (2 * h)
+    assert abs(fd - gk) <= FD_TOL * abs(fd), (fd, gk)
+
+
+def test_adjoint_rejects_mixed_context_and_bad_schedule():
+    P = params()
+    phi0, target, _ = case(P)
+    mx = hip.InamuroHIP(P, precision=hip.MIXED)
+    with pytest.raises(Exception, match="FP64"):
+        mx.adjoint(phi0, target, 1)
+    sim = hip.InamuroHIP(P)
+    with pytest.raises(Exception, match="multiples"):
+        sim.adjoint(phi0, target, 2, iterations=[100, 150])
