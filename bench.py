@@ -96,6 +96,60 @@ def run_precision(hip, L, R, precision, steps, warmup):
     return out
 
 
+# Adjoint workload: 96^3 oscillating ellipsoid, semi-axes 20, 16, 16, the
+# static droplet's physics, J = sum (phi_T - phi_target)^2 with phi_target a
+# sphere of R = 17.2 (same tanh profile, width 4).
+ADJ = dict(L=96, axes=(20.0, 16.0, 16.0), R_target=17.2)
+
+
+def ellipsoid_phi(L, axes, width=2.0):
+    """phi0 = mean + (phi_L - phi_G)/2 tanh(d / width), with d the
+    first-order signed distance to the ellipsoid, (1 - s) / |grad s|."""
+    import numpy as np
+
+    c = L / 2.0
+    z, y, x = np.meshgrid(*(np.arange(L, dtype=np.float64) - c for _ in range(3)), indexing="ij")
+    a, b, cc = axes
+    s = np.sqrt((x / a) ** 2 + (y / b) ** 2 + (z / cc) ** 2)
+    gs = np.sqrt((x / a ** 2) ** 2 + (y / b ** 2) ** 2 + (z / cc ** 2) ** 2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d = np.where(s > 0, (1.0 - s) * s / np.where(gs > 0, gs, 1.0), min(axes))
+    P = PHYSICS
+    return 0.5 * (P["fei_L"] + P["fei_G"]) + 0.5 * (P["fei_L"] - P["fei_G"]) * np.tanh(d / width)
+
+
+def run_adjoint(hip, steps, checkpoint):
+    """Plain forward of `steps` steps, then lbm_adjoint of the same map with a
+    checkpoint every `checkpoint` steps; cost ratio = reverse sweep (replays
+    and recomputation included) over the plain forward."""
+    L = ADJ["L"]
+    P = hip.SolverParams(lx=L, ly=L, lz=L, DD=2.0 * ADJ["R_target"], **PHYSICS)
+    phi0 = ellipsoid_phi(L, ADJ["axes"])
+    target = ellipsoid_phi(L, (ADJ["R_target"],) * 3)
+    sim = hip.InamuroHIP(P)
+    sim.initFromCPU(fei=phi0, rho=sim.initial_rho(phi0), u=0.0, v=0.0, w=0.0, p=0.0)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    sim.performTimeStepGPU(steps)
+    sim.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3
+    iters = sim.iteration_history()
+    print(f"adjoint leg: plain forward {fwd_ms:.0f} ms, iterations {iters}", file=sys.stderr, flush=True)
+    J, g, gk = sim.adjoint(phi0, target, steps, checkpoint_interval=checkpoint)
+    st = sim.adjoint_stats()
+    sim.close()
+    return dict(
+        workload=f"ellipsoid_96: 96^3, semi-axes 20/16/16 -> target sphere R=17.2, FP64, {steps} steps from rest, "
+                 f"checkpoint every {checkpoint}",
+        steps=steps, checkpoint_interval=checkpoint, poisson_iterations_per_step=iters,
+        forward_ms=fwd_ms, adjoint_forward_ms=st.forward_ms, reverse_ms=st.reverse_ms,
+        adjoint_over_forward=st.reverse_ms / fwd_ms,
+        total_over_forward=(st.forward_ms + st.reverse_ms) / fwd_ms,
+        recomputed_iterations=int(st.recomputed_iterations), adjoint_iterations=int(st.adjoint_iterations),
+        store_slots=int(st.store_slots), J=J, dJ_dkg=gk, grad_phi0_linf=float(abs(g).max()),
+    )
+
+
 def cpu_baseline(L, R, n_it_per_step):
     """Single-thread CPU oracle on the same 256^3 state: one non-Poisson step
     part plus two Poisson iterations, extrapolated with the GPU's measured
@@ -136,6 +190,11 @@ def main(argv=None):
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--workload", choices=sorted(WORKLOADS), default="static_droplet_256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="run only the adjoint leg and print its record")
+    ap.add_argument("--adjoint-steps", type=int, default=8)
+    ap.add_argument("--adjoint-checkpoint", type=int, default=4)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one short solve per precision (for rocprofv3 --pmc runs) and exit")
     args = ap.parse_args(argv)
@@ -146,6 +205,9 @@ def main(argv=None):
 
     from lbm_amd import hip
 
+    if args.adjoint_only:
+        print(json.dumps(run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint)), flush=True)
+        return 0
     wl = WORKLOADS[args.workload]
     L, R = wl["L"], wl["R"]
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
@@ -212,6 +274,8 @@ def main(argv=None):
                 line["roofline"]["traffic"] = tb
             else:
                 line[f"{prec}_traffic_bytes_per_launch"] = tb
+    if not args.no_adjoint:
+        line["adjoint"] = run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(L, R, n_it_mean)
     print(json.dumps(line), flush=True)

@@ -403,8 +403,15 @@ void poisson_check(Ctx& c)
     lbm::f64::check_finalize(s, parts_err, parts_fp, nb, c.dual, c.tol, c.dst);
 }
 
+// Called after every completed convergence check with the iteration count
+// (the adjoint's replays record raw-h snapshots through it).
+struct CheckHook {
+    void (*fn)(Ctx&, uint64_t, void*);
+    void* arg;
+};
+
 // One performTimeStep.
-void step_one(Ctx& c)
+void step_one(Ctx& c, const CheckHook* hook = nullptr)
 {
     hipStream_t s = c.stream;
     step_fg(c);
@@ -475,6 +482,7 @@ void step_one(Ctx& c)
         }
         d.converged = st.converged;
         if (st.status != lbm::ST_OK) throw ApiError{LBM_ERR_NUMERIC, status_message(st.status)};
+        if (hook) hook->fn(c, iter, hook->arg);
         if (forced != 0) {
             if (iter >= forced) break;
             hmode = lbm::H_GAUGED;
@@ -592,15 +600,22 @@ void ensure_scratch(Ctx& c)
 // ---- discrete adjoint (no counterpart in the reference) ---------------------
 //
 // Checkpoint-and-recompute reverse sweep of the FP64 step with frozen Poisson
-// iteration counts:
-//   forward pass      state saved every K steps (37 doubles per cell)
-//   per segment       replay from its checkpoint, keeping each step's start
-//   per step, reverse recompute collide + macro + the Poisson solve, recording
-//                     its iterates (L slots; longer solves are split into
-//                     blocks of L iterations restarted from raw-h snapshots),
-//                     then the adjoint kernels of kernels_adjoint.hip.
-// The recorded iterate of iteration k is the raw streamed h it writes, so
-// recording costs no traffic beyond the forward kernel's own.
+// iteration counts.
+//   forward pass   the full state every K steps (37 doubles per cell), the
+//                  solve's final pressure of every step (1 per cell), and,
+//                  while a snapshot pool has room, the raw h / gauged p /
+//                  shift at iterations L, 2L, ... of solves longer than L
+//   per segment    replay from its checkpoint without the Poisson solves:
+//                  collide + macro, the archived final p, correct_uvw give
+//                  the next step's start bit for bit
+//   per step,      recompute collide + macro; then, last block first,
+//   reverse        recompute the block's L Poisson iterations recording every
+//                  iterate and run their adjoints (kernels_adjoint.hip);
+//                  then the macro, collision and stencil adjoints.
+// The recorded iterate of iteration k is the raw streamed h the forward
+// kernel writes anyway, so recording adds no traffic.  A long solve without
+// recorded snapshots (pool exhausted) recomputes each block's start from the
+// beginning of the solve instead.
 
 constexpr size_t STATE_PER_CELL = 7 + 2 * lbm::Q;  // fei rho u v w p p_prev, f, g
 
@@ -613,19 +628,25 @@ struct StepState {
 struct AdjWork {
     std::vector<StepState> ckpt, seg;
     StepState final_state;
-    double* slots = nullptr;  // L raw-h slots, 15n each
-    std::vector<double*> snaps;
-    double* pstore = nullptr;  // gauged p of every check, n each
-    double* shifts = nullptr;  // gauge shift of every check
+    double* pfinal = nullptr;     // final p of every step, T x n
+    uint64_t L = 0;               // block length (recorded iterates)
+    double* slots = nullptr;      // max(L, 2) raw-h slots, 15n each
+    double* pstore = nullptr;     // gauged p of the block's checks, (L/ci + 1) x n
+    double* shifts = nullptr;     // their shifts
+    double* snap_pool = nullptr;  // snapshots: raw h (15n), gauged p (n), shift
+    double* scratch_snap = nullptr;  // one more, for solves whose snapshots overflowed
+    size_t snap_stride = 0;       // doubles per snapshot
+    size_t snap_cap = 0, snap_next = 0;
+    std::vector<std::vector<double*>> step_snaps;  // per step; empty = not recorded
+    std::vector<char> have_snaps;
     double *lfei = nullptr, *lrho = nullptr, *lu = nullptr, *lv = nullptr, *lw = nullptr;
     double *lp = nullptr, *lps = nullptr, *lpf = nullptr, *lS = nullptr, *ltau = nullptr;
     double *lmac = nullptr, *pc = nullptr, *parts = nullptr, *target = nullptr;
-    double* L[2] = {nullptr, nullptr};
+    double* Lb[2] = {nullptr, nullptr};
     double* cf[2] = {nullptr, nullptr};
     double* cg[2] = {nullptr, nullptr};
     lbm::AdjState* as = nullptr;
     lbm::AdjState* has = nullptr;
-    uint64_t L_slots = 0;
 
     ~AdjWork()
     {
@@ -636,17 +657,28 @@ struct AdjWork {
         for (auto& s : ckpt) F(s.buf);
         for (auto& s : seg) F(s.buf);
         F(final_state.buf);
-        F(slots);
-        for (auto& p : snaps) F(p);
-        F(pstore); F(shifts);
+        F(pfinal); F(slots); F(pstore); F(shifts); F(snap_pool); F(scratch_snap);
         for (double** p : {&lfei, &lrho, &lu, &lv, &lw, &lp, &lps, &lpf, &lS, &ltau, &lmac, &pc, &parts, &target})
             F(*p);
-        for (int i = 0; i < 2; ++i) { F(L[i]); F(cf[i]); F(cg[i]); }
+        for (int i = 0; i < 2; ++i) { F(Lb[i]); F(cf[i]); F(cg[i]); }
         F(as);
         if (has) (void)hipHostFree(has);
     }
     double* slot(const Ctx& c, uint64_t m) const { return slots + (m - 1) * lbm::Q * c.n(); }
 };
+
+// A snapshot buffer: raw h at offset 0, gauged p at 15n, shift at 16n.
+struct SnapView {
+    double* h;
+    double* p;
+    double* shift;
+};
+
+SnapView snap_view(const Ctx& c, double* base)
+{
+    const size_t n = c.n();
+    return SnapView{base, base + lbm::Q * n, base + (lbm::Q + 1) * n};
+}
 
 void d2d(Ctx& c, void* dst, const void* src, size_t bytes)
 {
@@ -677,32 +709,100 @@ void restore_state(Ctx& c, const StepState& st)
     c.hs = st.hs;
 }
 
+// Forward-pass hook: after the check at iteration L*j keep raw h, the gauged
+// p and the shift in the step's next snapshot, while the pool has room.
+struct SnapRecorder {
+    AdjWork* a;
+    std::vector<double*>* snaps;
+    bool overflow = false;
+};
+
+void record_snapshot(Ctx& c, uint64_t iter, void* arg)
+{
+    SnapRecorder& r = *static_cast<SnapRecorder*>(arg);
+    if (iter % r.a->L != 0 || r.overflow) return;
+    if (r.a->snap_next >= r.a->snap_cap) {
+        r.overflow = true;
+        return;
+    }
+    double* base = r.a->snap_pool + (r.a->snap_next++) * r.a->snap_stride;
+    r.snaps->push_back(base);
+    const SnapView v = snap_view(c, base);
+    const size_t nb = c.n() * sizeof(double);
+    d2d(c, v.h, c.h[c.hcur], lbm::Q * nb);
+    d2d(c, v.p, c.p[c.pc], nb);
+    d2d(c, v.shift, &c.dst->shift, sizeof(double));
+}
+
+// A step of a segment replay: collide + macro, the archived final pressure
+// (also the last check's p_prev) and the velocity correction reproduce what
+// step_one leaves behind without running the solve.
+void step_light(Ctx& c, const double* p_final)
+{
+    step_fg(c);
+    const size_t nb = c.n() * sizeof(double);
+    d2d(c, c.p[c.pc], p_final, nb);
+    d2d(c, c.p_prev, p_final, nb);
+    lbm::f64::correct(c.stream, c.g, c.p[c.pc], D<double>(c.rho), D<double>(c.u), D<double>(c.v),
+                      D<double>(c.w));
+    HIPCHK(hipGetLastError());
+    c.hs = HState::Equilibrium;
+}
+
 // Recompute Poisson iterations a0+1..a1 of the current step (collide + macro
-// already run).  Block mode writes iteration k's raw h into slot k-a0; the
-// snapshot pass (a0 = 0, a1 = nit) ping-pongs through slots 1 and 2 and keeps
-// the raw h of iterations L, 2L, ... in the snapshot buffers.  Every check
-// stores its gauged p and shift.
-void adj_forward(Ctx& c, AdjWork& a, const double* p_start, uint64_t a0, uint64_t a1, bool snapshot_pass)
+// already run), writing iteration k's raw h into slot k - a0 and the gauged p
+// and shift of each check into the block-local store.  For a0 > 0 the block
+// starts from snapshot `start` (raw h, gauged p and shift of check a0).
+void adj_forward_block(Ctx& c, AdjWork& a, const double* p_start, double* start, uint64_t a0, uint64_t a1)
 {
     const uint64_t ci = static_cast<uint64_t>(c.check_interval);
     const size_t n = c.n();
     int mode = lbm::H_EQUILIBRIUM;
     const double* in = a.slot(c, 1);  // not read in the equilibrium mode
+    const double* plast = p_start;
     if (a0 > 0) {
+        const SnapView v = snap_view(c, start);
         mode = lbm::H_GAUGED;
-        in = a.snaps[a0 / a.L_slots - 1];
-        d2d(c, &c.dst->shift, a.shifts + a0 / ci, sizeof(double));
+        in = v.h;
+        plast = v.p;
+        d2d(c, &c.dst->shift, v.shift, sizeof(double));
     }
     for (uint64_t k = a0 + 1; k <= a1; ++k) {
-        double* out;
-        if (!snapshot_pass) out = a.slot(c, k - a0);
-        else if (k % a.L_slots == 0 && k < a1) out = a.snaps[k / a.L_slots - 1];
-        else out = a.slot(c, 1 + (k & 1));
-        const double* pbuf = mode == lbm::H_GAUGED ? a.pstore + ((k - 1) / ci) * n : p_start;
-        lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, pbuf, D<double>(c.tau), D<double>(c.src), c.dst);
+        double* out = a.slot(c, k - a0);
+        lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, plast, D<double>(c.tau), D<double>(c.src), c.dst);
         if (k % ci == 0) {
-            lbm::f64::gauge(c.stream, c.g, c.ph, out, a.pstore + (k / ci) * n, c.p_prev, c.partials, c.dst);
-            d2d(c, a.shifts + k / ci, &c.dst->shift, sizeof(double));
+            double* pk = a.pstore + ((k - a0) / ci) * n;
+            lbm::f64::gauge(c.stream, c.g, c.ph, out, pk, c.p_prev, c.partials, c.dst);
+            d2d(c, a.shifts + (k - a0) / ci, &c.dst->shift, sizeof(double));
+            plast = pk;
+            mode = lbm::H_GAUGED;
+        } else {
+            mode = lbm::H_STREAMED;
+        }
+        in = out;
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// Recompute iterations 1..a1 of the current step through slots 1 and 2 and
+// leave the state of check a1 in snapshot `dst_snap` (for a solve whose
+// snapshots were not recorded).
+void adj_forward_prefix(Ctx& c, AdjWork& a, const double* p_start, double* dst_snap, uint64_t a1)
+{
+    const uint64_t ci = static_cast<uint64_t>(c.check_interval);
+    const size_t n = c.n();
+    const SnapView v = snap_view(c, dst_snap);
+    int mode = lbm::H_EQUILIBRIUM;
+    const double* in = a.slot(c, 1);  // not read in the equilibrium mode
+    const double* plast = p_start;
+    for (uint64_t k = 1; k <= a1; ++k) {
+        double* out = k == a1 ? v.h : a.slot(c, 1 + (k & 1));
+        lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, plast, D<double>(c.tau), D<double>(c.src), c.dst);
+        if (k % ci == 0) {
+            double* pk = k == a1 ? v.p : a.pstore + ((k / ci) & 1) * n;
+            lbm::f64::gauge(c.stream, c.g, c.ph, out, pk, c.p_prev, c.partials, c.dst);
+            if (k == a1) d2d(c, v.shift, &c.dst->shift, sizeof(double));
+            plast = pk;
             mode = lbm::H_GAUGED;
         } else {
             mode = lbm::H_STREAMED;
@@ -713,13 +813,15 @@ void adj_forward(Ctx& c, AdjWork& a, const double* p_start, uint64_t a0, uint64_
 }
 
 // Reverse of one step whose start state is st (already restored into c).
-void adj_step(Ctx& c, AdjWork& a, const StepState& st, uint64_t nit, int ccur, lbm_adjoint_stats_t& stats)
+// snaps: the forward pass's snapshots of this solve, if it recorded them.
+void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*>& snaps, bool have_snaps,
+              uint64_t nit, int ccur, lbm_adjoint_stats_t& stats)
 {
     hipStream_t s = c.stream;
     const Grid& g = c.g;
     const size_t n = c.n();
     const uint64_t ci = static_cast<uint64_t>(c.check_interval);
-    const uint64_t L = a.L_slots;
+    const uint64_t L = a.L;
     const double* s_fei = st.buf;
     const double* s_rho = st.buf + n;
     const double* s_u = st.buf + 2 * n;
@@ -730,33 +832,45 @@ void adj_step(Ctx& c, AdjWork& a, const StepState& st, uint64_t nit, int ccur, l
 
     step_fg(c);  // fei', rho', u', tau_h, S
     const uint64_t nblocks = (nit + L - 1) / L;
-    if (nblocks == 1) adj_forward(c, a, p_start, 0, nit, false);
-    else adj_forward(c, a, p_start, 0, nit, true);
-    stats.recomputed_iterations += nit;
-    const double* p_final = a.pstore + (nit / ci) * n;
-
-    lbm::adj::correct(s, g, c.ph, p_final, D<double>(c.rho), a.lu, a.lv, a.lw, a.lp, a.lrho, a.lpf, a.parts, a.as);
     HIPCHK(hipMemsetAsync(a.lS, 0, n * sizeof(double), s));
     HIPCHK(hipMemsetAsync(a.ltau, 0, n * sizeof(double), s));
     int lcur = 0;
     for (uint64_t j = nblocks; j-- > 0;) {
         const uint64_t a0 = j * L, a1 = std::min(nit, a0 + L);
-        if (nblocks > 1) {
-            adj_forward(c, a, p_start, a0, a1, false);
-            stats.recomputed_iterations += a1 - a0;
+        double* start = nullptr;
+        if (j > 0) {
+            if (have_snaps) {
+                start = snaps[j - 1];
+            } else {
+                adj_forward_prefix(c, a, p_start, a.scratch_snap, a0);
+                stats.recomputed_iterations += a0;
+                start = a.scratch_snap;
+            }
         }
+        adj_forward_block(c, a, p_start, start, a0, a1);
+        stats.recomputed_iterations += a1 - a0;
+        if (j == nblocks - 1) {
+            // correct_uvw reads the solve's final (gauged) p.
+            const double* p_final = a.pstore + ((nit - a0) / ci) * n;
+            lbm::adj::correct(s, g, c.ph, p_final, D<double>(c.rho), a.lu, a.lv, a.lw, a.lp, a.lrho, a.lpf,
+                              a.parts, a.as);
+        }
+        const SnapView sv = start ? snap_view(c, start) : SnapView{nullptr, nullptr, nullptr};
         for (uint64_t k = a1; k > a0; --k) {
             const int src = k == nit ? lbm::ADJ_SRC_INIT : (k % ci == 0 ? lbm::ADJ_SRC_LAZY : lbm::ADJ_SRC_PLAIN);
             const int mode = k == 1 ? lbm::H_EQUILIBRIUM
                                     : ((k - 1) % ci == 0 ? lbm::H_GAUGED : lbm::H_STREAMED);
-            const double* raw = nullptr;
-            if (k >= a0 + 2) raw = a.slot(c, k - 1 - a0);
-            else if (a0 > 0) raw = a.snaps[a0 / L - 1];
-            const double* pin = mode == lbm::H_EQUILIBRIUM ? p_start
-                               : (mode == lbm::H_GAUGED ? a.pstore + ((k - 1) / ci) * n : nullptr);
-            const double* shift = mode == lbm::H_GAUGED ? a.shifts + (k - 1) / ci : nullptr;
-            double* out = k == 1 ? a.lps : a.L[1 - lcur];
-            lbm::adj::poisson(s, g, c.ph, src, mode, a.L[lcur], a.lpf, a.as, raw, pin, shift,
+            const double* raw = k >= a0 + 2 ? a.slot(c, k - 1 - a0) : sv.h;
+            const double* pin = nullptr;
+            const double* shift = nullptr;
+            if (mode == lbm::H_EQUILIBRIUM) {
+                pin = p_start;
+            } else if (mode == lbm::H_GAUGED) {
+                pin = k - 1 == a0 ? sv.p : a.pstore + ((k - 1 - a0) / ci) * n;
+                shift = k - 1 == a0 ? sv.shift : a.shifts + (k - 1 - a0) / ci;
+            }
+            double* out = k == 1 ? a.lps : a.Lb[1 - lcur];
+            lbm::adj::poisson(s, g, c.ph, src, mode, a.Lb[lcur], a.lpf, a.as, raw, pin, shift,
                               D<double>(c.tau), a.lS, a.ltau, out, a.parts);
             if (k > 1) lcur = 1 - lcur;
         }
@@ -772,17 +886,6 @@ void adj_step(Ctx& c, AdjWork& a, const StepState& st, uint64_t nit, int ccur, l
     std::swap(a.lp, a.lps);
 }
 
-// Largest block length L (a multiple of the check interval, at most the
-// longest solve) whose slots and snapshots fit in `slots_fit` buffers of 15n.
-uint64_t choose_block(uint64_t max_nit, uint64_t ci, uint64_t slots_fit)
-{
-    for (uint64_t L = (std::min(max_nit, slots_fit) / ci) * ci; L >= ci; L -= ci) {
-        const uint64_t snaps = (max_nit + L - 1) / L - 1;
-        if (std::max<uint64_t>(L, 2) + snaps <= slots_fit) return L;
-    }
-    return 0;
-}
-
 void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K, const uint64_t* iters,
                  double* J, double* grad_phi0, double* grad_kg)
 {
@@ -790,10 +893,13 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     if (!phi0 || !target || !J || T < 1 || K < 1)
         throw ApiError{LBM_ERR_ARG, "adjoint needs phi0, target, objective, steps >= 1 and checkpoint interval >= 1"};
     const uint64_t ci = static_cast<uint64_t>(c.check_interval);
+    uint64_t sched_max = 0;
     if (iters)
-        for (int k = 0; k < T; ++k)
+        for (int k = 0; k < T; ++k) {
             if (iters[k] == 0 || iters[k] % ci != 0)
                 throw ApiError{LBM_ERR_ARG, "scheduled iteration counts must be positive multiples of the check interval"};
+            sched_max = std::max(sched_max, iters[k]);
+        }
     hipStream_t s = c.stream;
     const Grid& g = c.g;
     const size_t n = c.n();
@@ -802,6 +908,54 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     stats.steps = T;
     stats.checkpoint_interval = K;
     AdjWork a;
+
+    // Everything is allocated before the forward pass.
+    const int nseg = (T + K - 1) / K;
+    a.ckpt.resize(static_cast<size_t>(nseg));
+    for (auto& st : a.ckpt) alloc_state(c, st);
+    alloc_state(c, a.final_state);
+    a.seg.resize(static_cast<size_t>(std::min(K, T)));
+    for (auto& st : a.seg) alloc_state(c, st);
+    a.pfinal = dalloc(static_cast<size_t>(T) * n);
+    for (double** p : {&a.lfei, &a.lrho, &a.lu, &a.lv, &a.lw, &a.lp, &a.lps, &a.lpf, &a.lS, &a.ltau, &a.lmac, &a.target})
+        *p = dalloc(n);
+    for (int i = 0; i < 2; ++i) {
+        a.Lb[i] = dalloc(lbm::Q * n);
+        a.cf[i] = dalloc(lbm::Q * n);
+        a.cg[i] = dalloc(lbm::Q * n);
+    }
+    a.pc = dalloc(lbm::adj::PC_SLOTS * n);
+    a.parts = dalloc(static_cast<size_t>(lbm::grid_blocks(g.n)));
+    a.as = static_cast<lbm::AdjState*>(dalloc_bytes(sizeof(lbm::AdjState)));
+    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&a.has), sizeof(lbm::AdjState), hipHostMallocDefault));
+
+    // Iterate memory: three quarters for the block's recorded iterates (and
+    // its check store), one quarter for the snapshot pool.
+    size_t budget = static_cast<size_t>(std::max<int64_t>(c.adj_store_bytes, 0));
+    if (budget == 0) {
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        const size_t reserve = size_t(2) << 30;
+        budget = free_b > reserve ? (free_b - reserve) / 10 * 9 : 0;
+    }
+    const size_t slot_d = lbm::Q * n;
+    a.snap_stride = ((lbm::Q + 1) * n + 1 + 31) / 32 * 32;
+    const size_t pool_d = budget / 4 / sizeof(double);
+    const size_t slots_d = budget / sizeof(double) - pool_d;
+    uint64_t L = (static_cast<uint64_t>(slots_d / (slot_d + n / ci + 1)) / ci) * ci;
+    if (sched_max) L = std::min(L, sched_max);
+    while (L >= ci && std::max<uint64_t>(L, 2) * slot_d + (L / ci + 1) * (n + 1) > slots_d) L -= ci;
+    if (L < ci) throw ApiError{LBM_ERR_NOMEM, "adjoint needs device memory for check_interval recorded Poisson iterates"};
+    a.L = L;
+    stats.store_slots = static_cast<int64_t>(L);
+    a.slots = dalloc(std::max<uint64_t>(L, 2) * slot_d);
+    a.pstore = dalloc((L / ci + 1) * n);
+    a.shifts = dalloc(L / ci + 1);
+    a.scratch_snap = dalloc(a.snap_stride);
+    a.snap_cap = pool_d > a.snap_stride ? (pool_d - a.snap_stride) / a.snap_stride : 0;
+    if (a.snap_cap) a.snap_pool = dalloc(a.snap_cap * a.snap_stride);
+    a.step_snaps.resize(static_cast<size_t>(T));
+    a.have_snaps.assign(static_cast<size_t>(T), 0);
 
     // Initial state: rho0 linear in phi0 (place_droplet's rule), rest zero.
     HostFields h0;
@@ -819,15 +973,27 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     else c.schedule.clear();
     c.sched_pos = 0;
 
-    const int nseg = (T + K - 1) / K;
-    a.ckpt.resize(static_cast<size_t>(nseg));
-    for (auto& st : a.ckpt) alloc_state(c, st);
-    alloc_state(c, a.final_state);
+    // Forward pass.
     sync(c);
     auto t0 = std::chrono::steady_clock::now();
     for (int k = 0; k < T; ++k) {
+        const size_t ks = static_cast<size_t>(k);
         if (k % K == 0) save_state(c, a.ckpt[static_cast<size_t>(k / K)]);
-        step_one(c);
+        const size_t pool_mark = a.snap_next;
+        SnapRecorder rec{&a, &a.step_snaps[ks]};
+        const CheckHook hook{record_snapshot, &rec};
+        step_one(c, &hook);
+        const uint64_t nit = c.history.back();
+        const size_t need = static_cast<size_t>((nit + L - 1) / L - 1);
+        if (rec.overflow) {
+            a.step_snaps[ks].clear();  // incomplete: this step takes its own snapshot pass
+            a.snap_next = pool_mark;
+        } else {
+            a.step_snaps[ks].resize(need);  // drop a snapshot taken at the final check
+            a.snap_next = pool_mark + need;
+            a.have_snaps[ks] = 1;
+        }
+        d2d(c, a.pfinal + ks * n, c.p[c.pc], n * sizeof(double));
     }
     save_state(c, a.final_state);
     sync(c);
@@ -835,50 +1001,14 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     stats.forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     const std::vector<uint64_t> sched(c.history.end() - T, c.history.end());
     const std::vector<uint64_t> history = c.history;
-    uint64_t max_nit = 0;
-    for (uint64_t v : sched) {
-        max_nit = std::max(max_nit, v);
-        stats.forward_iterations += v;
-    }
-
-    // Reverse-sweep buffers.
-    a.seg.resize(static_cast<size_t>(std::min(K, T)));
-    for (auto& st : a.seg) alloc_state(c, st);
-    for (double** p : {&a.lfei, &a.lrho, &a.lu, &a.lv, &a.lw, &a.lp, &a.lps, &a.lpf, &a.lS, &a.ltau, &a.lmac, &a.target})
-        *p = dalloc(n);
-    for (int i = 0; i < 2; ++i) {
-        a.L[i] = dalloc(lbm::Q * n);
-        a.cf[i] = dalloc(lbm::Q * n);
-        a.cg[i] = dalloc(lbm::Q * n);
-    }
-    a.pc = dalloc(lbm::adj::PC_SLOTS * n);
-    a.parts = dalloc(static_cast<size_t>(lbm::grid_blocks(g.n)));
-    a.pstore = dalloc((max_nit / ci + 1) * n);
-    a.shifts = dalloc(max_nit / ci + 1);
-    a.as = static_cast<lbm::AdjState*>(dalloc_bytes(sizeof(lbm::AdjState)));
-    HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&a.has), sizeof(lbm::AdjState), hipHostMallocDefault));
-    const size_t slot_bytes = lbm::Q * n * sizeof(double);
-    size_t budget = static_cast<size_t>(std::max<int64_t>(c.adj_store_bytes, 0));
-    if (budget == 0) {
-        size_t free_b = 0, total_b = 0;
-        HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        const size_t reserve = size_t(2) << 30;
-        budget = free_b > reserve ? (free_b - reserve) / 10 * 9 : 0;
-    }
-    const uint64_t L = choose_block(max_nit, ci, budget / slot_bytes);
-    if (L == 0) throw ApiError{LBM_ERR_NOMEM, "adjoint needs device memory for check_interval recorded Poisson iterates"};
-    a.L_slots = L;
-    stats.store_slots = static_cast<int64_t>(L);
-    a.slots = static_cast<double*>(dalloc_bytes(std::max<uint64_t>(L, 2) * slot_bytes));
-    a.snaps.resize((max_nit + L - 1) / L - 1, nullptr);
-    for (auto& p : a.snaps) p = dalloc(lbm::Q * n);
+    for (uint64_t v : sched) stats.forward_iterations += v;
 
     // Seed: J and dJ/dfei_T; every other adjoint starts at zero.
     HIPCHK(hipMemcpyAsync(a.target, target, n * sizeof(double), hipMemcpyHostToDevice, s));
     lbm::adj::seed(s, g, D<double>(c.fei), a.target, a.lfei, a.parts, a.as);
     for (double* p : {a.lrho, a.lu, a.lv, a.lw, a.lp}) HIPCHK(hipMemsetAsync(p, 0, n * sizeof(double), s));
-    HIPCHK(hipMemsetAsync(a.cf[0], 0, slot_bytes, s));
-    HIPCHK(hipMemsetAsync(a.cg[0], 0, slot_bytes, s));
+    HIPCHK(hipMemsetAsync(a.cf[0], 0, slot_d * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(a.cg[0], 0, slot_d * sizeof(double), s));
     HIPCHK(hipGetLastError());
 
     const bool timing = c.timing;
@@ -889,15 +1019,15 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     for (int sg = nseg - 1; sg >= 0; --sg) {
         const int k0 = sg * K, k1 = std::min(T, k0 + K);
         restore_state(c, a.ckpt[static_cast<size_t>(sg)]);
-        c.sched_pos = static_cast<size_t>(k0);
         for (int k = k0; k < k1; ++k) {
             save_state(c, a.seg[static_cast<size_t>(k - k0)]);
-            if (k + 1 < k1) step_one(c);
+            if (k + 1 < k1) step_light(c, a.pfinal + static_cast<size_t>(k) * n);
         }
         for (int k = k1 - 1; k >= k0; --k) {
+            const size_t ks = static_cast<size_t>(k);
             const StepState& st = a.seg[static_cast<size_t>(k - k0)];
             restore_state(c, st);
-            adj_step(c, a, st, sched[static_cast<size_t>(k)], ccur, stats);
+            adj_step(c, a, st, a.step_snaps[ks], a.have_snaps[ks] != 0, sched[ks], ccur, stats);
             ccur = 1 - ccur;
         }
     }

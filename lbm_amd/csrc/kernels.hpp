@@ -1,5 +1,6 @@
 // kernels.hpp -- launchers shared by the host orchestration (capi.cpp) and the
-// kernel translation units (kernels_f64.hip, kernels_mixed.hip).
+// kernel translation units (kernels_f64.hip, kernels_mixed.hip,
+// kernels_adjoint.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -122,9 +123,9 @@ void seed(hipStream_t s, const Grid& g, const double* fei, const double* target,
 void correct(hipStream_t s, const Grid& g, const Phys& ph, const double* p, const double* rho,
              const double* lu, const double* lv, const double* lw, const double* lp, double* lrho,
              double* lpf, double* parts, AdjState* as);
-void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, const double* Lin,
+void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, const double* Bin,
              const double* lpf, AdjState* as, const double* raw, const double* pin,
-             const double* shift, const double* tau, double* lS, double* ltau, double* Lout,
+             const double* shift, const double* tau, double* lS, double* ltau, double* Bout,
              double* parts);
 void source(hipStream_t s, const Grid& g, const Phys& ph, const double* fei, const double* rho,
             const double* ltau, const double* lS, const double* lfei, AdjState* as, double* lrho,

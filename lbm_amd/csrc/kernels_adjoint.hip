@@ -7,9 +7,9 @@
 //   forward                                 adjoint here
 //   correct_uvw (CPU/Inamuro.cpp:794-823)   k_adj_correct    gather of the
 //                                           mirrored / periodic differences
-//   Poisson iteration, push form            k_adj_poisson    gather (pull) of
-//   (collide, stream, x-slip, getp,         the streamed adjoint with the slip
-//   lazy gauge; CPU/Inamuro.cpp:673-793)    transposed; lazy gauge adjoint
+//   Poisson iteration, push form            k_adj_poisson    push form too:
+//   (collide, stream, x-slip, getp,         own-cell loads, slip transposed
+//   lazy gauge; CPU/Inamuro.cpp:673-793)    in place, shifted stores; lazy gauge
 //   gauge p -= s, h -= E s                  k_adj_lam_finalize
 //   getMacro (CPU/Inamuro.cpp:294-346),     k_adj_source + the gather inside
 //   pull + slip of f, g                     k_adj_collide
@@ -17,9 +17,9 @@
 //   firstord / secondord                    k_adj_stencil   gather over the
 //   (CPU/Inamuro.hpp:213-314)               cells whose stencil reads x
 //
-// Every kernel is a gather: each thread owns one cell and writes only it, so
-// no atomics are needed and every result is deterministic.  Reductions use the
-// same fixed-shape wave64 trees as the forward path.  Compiled with
+// Every kernel is a gather or a permutation (each output written by exactly
+// one thread), so no atomics are needed and every result is deterministic.
+// Reductions use the same fixed-shape wave64 trees as the forward path.  Compiled with
 // -ffp-contract=off like the forward FP64 kernels.
 //
 // Data layout: as the forward path, macro fields [z][y][x] (x fastest), FP64,
@@ -205,25 +205,31 @@ __global__ __launch_bounds__(BLOCK) void k_adj_lam_finalize(const double* __rest
 
 enum { SRC_PLAIN = 0, SRC_LAZY = 1, SRC_INIT = 2 };
 
-// Adjoint of forward iteration k.  Lin holds the adjoint of h_k as the next
-// iteration saw it (slip view, pressure-sum adjoint folded in), or, for the
-// last iteration (SRC_INIT), every direction of a cell carries lpf.  With
-// SRC_LAZY / SRC_INIT the pending gauge adjoint -Lambda/N is applied on read.
+// Adjoint of forward iteration k, in push form (the transpose of the forward
+// push: loads at the cell itself, stores shifted).  Bin holds, at each cell x,
+// lh*_q(x): the adjoint of the value iteration k's collision produced there,
+// already streamed back from where it landed (x + c_q) and slip-transposed.
+// With SRC_LAZY the pending gauge adjoint of check k is applied on read: each
+// such value carries -Lambda/N once per viewed value it collects (0 for a
+// value the slip overwrote, 2 where it also fed its mirror partner).  For the
+// last iteration (SRC_INIT) every viewed value at y carries lpf(y), gathered.
 // MODE is how forward iteration k obtained its input (h_{k-1}, p_{k-1}):
-//   H_EQUILIBRIUM: h = E p_start (first iteration); writes lps = d J / d p_start
+//   H_EQUILIBRIUM: h = E p_start (first iteration); writes Bout = dJ/dp_start
 //   H_GAUGED:      h = view(raw) - E shift, p = gauged p of the check; block
-//                  partials of sum_q E_q Lout_q for that check's adjoint
+//                  partials of sum_q E_q L_q for that check's adjoint
 //   H_STREAMED:    h = view(raw), p = sum_q h_q
 // Forward collision: h* = h - (h - E p)/tau - (E/3) S, so
 //   lv = lh* (1 - 1/tau),  lp = sum lh* E / tau,  lS -= sum lh* E / 3,
-//   ltau += sum lh* (h - E p) / tau^2.
+//   ltau += sum lh* (h - E p) / tau^2,
+// and L = lv + lp is the adjoint of iteration k-1's viewed output; its slip
+// transpose at x is pushed to x - c_q for the next (earlier) iteration.
 template <int SRC, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
-    Grid g, const double* __restrict__ Lin, const double* __restrict__ lpf,
+    Grid g, const double* __restrict__ Bin, const double* __restrict__ lpf,
     const AdjState* __restrict__ as, const double* __restrict__ raw,
     const double* __restrict__ pin, const double* __restrict__ shift_ptr,
     const double* __restrict__ tau, double* __restrict__ lS, double* __restrict__ ltau,
-    double* __restrict__ Lout, double* __restrict__ parts)
+    double* __restrict__ Bout, double* __restrict__ parts)
 {
     __shared__ double lds[4];
     const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
@@ -231,14 +237,25 @@ __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
     if (c < g.n) {
         int x, y, z;
         cell_xyz(c, g, x, y, z);
-        const double ls = (SRC == SRC_PLAIN) ? 0.0 : as->lam_shift;
         double lh[Q];
         if (SRC == SRC_INIT) {
+            const double ls = as->lam_shift;
             gather_stream_slip_T(x, y, z, g, [&](int, int64_t d) { return lpf[d] - ls; }, lh);
-        } else if (SRC == SRC_LAZY) {
-            gather_stream_slip_T(x, y, z, g, [&](int q, int64_t d) { return Lin[q * g.n + d] - ls; }, lh);
         } else {
-            gather_stream_slip_T(x, y, z, g, [&](int q, int64_t d) { return Lin[q * g.n + d]; }, lh);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) lh[q] = Bin[q * g.n + c];
+            if (SRC == SRC_LAZY) {
+                const double ls = as->lam_shift;
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    // multiplicity of the viewed values behind lh*_q(x) at
+                    // y = x + c_q
+                    double m = 1.0;
+                    if (CX[q] == 1) m = x == g.lx - 1 ? 0.0 : (x == g.lx - 2 ? 2.0 : 1.0);
+                    if (CX[q] == -1) m = x == 0 ? 0.0 : (x == 1 ? 2.0 : 1.0);
+                    lh[q] -= m * ls;
+                }
+            }
         }
         double hv[Q];
         double p;
@@ -273,13 +290,38 @@ __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
             double b = np;
 #pragma unroll
             for (int q = 0; q < Q; ++q) b += EW[q] * lh[q];
-            Lout[c] = b;
+            Bout[c] = b;
         } else {
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                const double o = lh[q] + np;
-                Lout[q * g.n + c] = o;
-                part += EW[q] * o;
+                lh[q] += np;
+                part += EW[q] * lh[q];
+            }
+            // slip transpose at x: the overwritten value gets nothing, its
+            // source collects both
+            if (x == 0) {
+#pragma unroll
+                for (int r = 0; r < 5; ++r) {
+                    lh[SLIP_NEG[r]] += lh[SLIP_POS[r]];
+                    lh[SLIP_POS[r]] = 0.0;
+                }
+            }
+            if (x == g.lx - 1) {
+#pragma unroll
+                for (int r = 0; r < 5; ++r) {
+                    lh[SLIP_POS[r]] += lh[SLIP_NEG[r]];
+                    lh[SLIP_NEG[r]] = 0.0;
+                }
+            }
+            const int xm = wrap(x, -1, g.lx), xp = wrap(x, 1, g.lx);
+            const int ym = wrap(y, -1, g.ly), yp = wrap(y, 1, g.ly);
+            const int zm = wrap(z, -1, g.lz), zp = wrap(z, 1, g.lz);
+#pragma unroll
+            for (int q = 0; q < Q; ++q) {
+                const int sx = CX[q] == 1 ? xm : (CX[q] == -1 ? xp : x);
+                const int sy = CY[q] == 1 ? ym : (CY[q] == -1 ? yp : y);
+                const int sz = CZ[q] == 1 ? zm : (CZ[q] == -1 ? zp : z);
+                Bout[q * g.n + lin(sx, sy, sz, g)] = lh[q];
             }
         }
     }
@@ -579,14 +621,14 @@ void correct(hipStream_t s, const Grid& g, const Phys& ph, const double* p, cons
     k_adj_lam_finalize<<<1, BLOCK, 0, s>>>(parts, nb, ph.cell_count, 1, as);
 }
 
-void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, const double* Lin,
+void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, const double* Bin,
              const double* lpf, AdjState* as, const double* raw, const double* pin,
-             const double* shift, const double* tau, double* lS, double* ltau, double* Lout,
+             const double* shift, const double* tau, double* lS, double* ltau, double* Bout,
              double* parts)
 {
     const int nb = grid_blocks(g.n);
 #define LBM_ADJ_LAUNCH(S, M)                                                                     \
-    k_adj_poisson<S, M><<<nb, BLOCK, 0, s>>>(g, Lin, lpf, as, raw, pin, shift, tau, lS, ltau, Lout, \
+    k_adj_poisson<S, M><<<nb, BLOCK, 0, s>>>(g, Bin, lpf, as, raw, pin, shift, tau, lS, ltau, Bout, \
                                              parts)
 #define LBM_ADJ_MODES(S)                                         \
     do {                                                         \
