@@ -92,8 +92,27 @@ def run_precision(hip, L, R, precision, steps, warmup):
         phase_ms=dict(collide=pf.collide_ms, macro=pf.macro_ms, poisson=pf.poisson_ms,
                       check=pf.check_ms, correct=pf.correct_ms),
     )
+    out["fields"] = sim.downloadFieldsToCPU()
     sim.close()
     return out
+
+
+def field_errors(mixed, fp64):
+    """Per-field error of the mixed-precision run against the FP64 run of the
+    same steps: L-inf and L2 relative to the FP64 field's max / L2 norm; the
+    spurious current is max |u| of a static droplet (ideally zero)."""
+    import numpy as np
+
+    err = {}
+    for k in ("fei", "u", "v", "w", "p"):
+        a, b = mixed[k], fp64[k]
+        d = a - b
+        err[k] = dict(linf_abs=float(np.abs(d).max()),
+                      linf_rel=float(np.abs(d).max() / max(np.abs(b).max(), 1e-300)),
+                      l2_rel=float(np.sqrt((d * d).sum()) / max(np.sqrt((b * b).sum()), 1e-300)))
+    speed = {name: float(np.sqrt(f["u"] ** 2 + f["v"] ** 2 + f["w"] ** 2).max())
+             for name, f in (("fp64", fp64), ("mixed", mixed))}
+    return err, speed
 
 
 # Adjoint workload: 96^3 oscillating ellipsoid, semi-axes 20, 16, 16, the
@@ -263,6 +282,10 @@ def main(argv=None):
         line["fp64_roofline_frac"] = results["fp64"]["achieved_gbs"] / HBM_PEAK_GBS
         line["mixed_over_fp64_mlups"] = results["mixed"]["mlups"] / results["fp64"]["mlups"]
         line["mixed_over_fp64_iteration_time"] = results["fp64"]["poisson_iter_ms"] / results["mixed"]["poisson_iter_ms"]
+        err, speed = field_errors(results["mixed"]["fields"], results["fp64"]["fields"])
+        line["mixed_vs_fp64_error"] = err
+        line["spurious_current_max_speed"] = speed
+        line["mixed_same_iterations"] = results["mixed"]["iterations"] == results["fp64"]["iterations"]
     # HBM bytes per launch of the dominant kernel, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy-kernel
     # calibration); null when no measurement exists for this workload.

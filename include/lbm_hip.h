@@ -170,6 +170,12 @@ int lbm_upload_dists(void* ctx, const double* f, const double* g, const double* 
 int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance,
                     uint64_t iteration_limit, int dual_residual);
 
+/* setUsePoissonGraph (GPU/InamuroCUDA.hpp:80, GPU/InamuroCUDA.cu:2550-2556):
+ * replay each check_interval segment of Poisson iterations as a captured HIP
+ * graph instead of launching its kernels one by one.  Results are identical
+ * either way; only launch overhead changes.  On by default here. */
+int lbm_set_poisson_graph(void* ctx, int enabled);
+
 /* Frozen Poisson iteration counts (no counterpart in the reference): step i
  * after this call runs exactly iterations[i] iterations, gauging at every
  * check, whatever the residual; later steps converge as usual.  This makes the

@@ -105,6 +105,16 @@ struct Ctx {
     int64_t adj_store_bytes = 0;     // device bytes for recorded Poisson iterates (0 = automatic)
     lbm_adjoint_stats_t adj_stats{};
 
+    // Captured Poisson segments (setUsePoissonGraph): one graph per entry
+    // mode and buffer parity, each holding check_interval iterations.
+    struct SegGraph {
+        int mode, hcur, pb, ci;
+        hipGraphExec_t exec;
+        int hcur_after, pb_after;
+    };
+    bool use_graphs = true;
+    std::vector<SegGraph> graphs;
+
     bool timing = false;
     lbm_perf_t perf{};
     hipEvent_t ev[10] = {};
@@ -137,6 +147,8 @@ void free_all(Ctx& c)
         if (e) (void)hipEventDestroy(e);
         e = nullptr;
     }
+    for (auto& gph : c.graphs) (void)hipGraphExecDestroy(gph.exec);
+    c.graphs.clear();
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
 }
@@ -380,6 +392,57 @@ void poisson_iteration(Ctx& c, int mode)
     c.hcur = 1 - c.hcur;
 }
 
+// check_interval iterations, the first in `mode`, the rest H_STREAMED:
+// eagerly, or by replaying a captured graph of exactly these launches
+// (buildPoissonGraphSegment, GPU/InamuroCUDA.cu:1482-1534).  Kernel nodes only.
+void poisson_segment(Ctx& c, int mode)
+{
+    if (!c.use_graphs) {
+        for (int k = 0; k < c.check_interval; ++k) {
+            poisson_iteration(c, mode);
+            mode = lbm::H_STREAMED;
+        }
+        return;
+    }
+    for (const auto& gph : c.graphs) {
+        if (gph.mode == mode && gph.hcur == c.hcur && gph.pb == c.pb && gph.ci == c.check_interval) {
+            HIPCHK(hipGraphLaunch(gph.exec, c.stream));
+            c.hcur = gph.hcur_after;
+            c.pb = gph.pb_after;
+            return;
+        }
+    }
+    Ctx::SegGraph gph{mode, c.hcur, c.pb, c.check_interval, nullptr, 0, 0};
+    HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+        int m = mode;
+        for (int k = 0; k < c.check_interval; ++k) {
+            poisson_iteration(c, m);
+            m = lbm::H_STREAMED;
+        }
+        HIPCHK(hipGetLastError());
+    } catch (...) {
+        (void)hipStreamEndCapture(c.stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        c.hcur = gph.hcur;
+        c.pb = gph.pb;
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(c.stream, &graph));
+    const hipError_t e = hipGraphInstantiate(&gph.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+        c.hcur = gph.hcur;
+        c.pb = gph.pb;
+        throw HipError{e, "hipGraphInstantiate"};
+    }
+    gph.hcur_after = c.hcur;
+    gph.pb_after = c.pb;
+    c.graphs.push_back(gph);
+    HIPCHK(hipGraphLaunch(gph.exec, c.stream));
+}
+
 // getp + gauge + relative change (+ dual residual) after a segment.
 void poisson_check(Ctx& c)
 {
@@ -434,13 +497,10 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
     bool first_segment = true;
     for (;;) {
         record(c, 4);
-        for (int k = 0; k < c.check_interval; ++k) {
-            if (iter == std::numeric_limits<uint64_t>::max())
-                throw ApiError{LBM_ERR_OVERFLOW, "Poisson iteration counter overflow"};
-            ++iter;
-            poisson_iteration(c, hmode);
-            hmode = lbm::H_STREAMED;
-        }
+        if (iter > std::numeric_limits<uint64_t>::max() - static_cast<uint64_t>(c.check_interval))
+            throw ApiError{LBM_ERR_OVERFLOW, "Poisson iteration counter overflow"};
+        iter += static_cast<uint64_t>(c.check_interval);
+        poisson_segment(c, hmode);
         HIPCHK(hipGetLastError());
         record(c, 5);
         poisson_check(c);
@@ -1233,6 +1293,14 @@ int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance, uint64_
         if (dual_residual < 0 || dual_residual > 2)
             throw ApiError{LBM_ERR_ARG, "dual_residual must be 0 (off), 1 (dual exit) or 2 (shadow)"};
         c.dual = dual_residual;
+    });
+}
+
+int lbm_set_poisson_graph(void* ctx, int enabled)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        // setUsePoissonGraph, GPU/InamuroCUDA.cu:2550-2556.
+        c.use_graphs = enabled != 0;
     });
 }
 

@@ -42,7 +42,7 @@ EXPORTED = (
     "lbm_poisson_diag", "lbm_synchronize", "lbm_set_timing", "lbm_perf_get",
     "lbm_perf_reset", "lbm_mass", "lbm_init_reference", "lbm_place_droplet",
     "lbm_reset_fields", "lbm_set_iteration_schedule", "lbm_iteration_history",
-    "lbm_adjoint", "lbm_set_adjoint_memory", "lbm_adjoint_stats",
+    "lbm_adjoint", "lbm_set_adjoint_memory", "lbm_adjoint_stats", "lbm_set_poisson_graph",
 )
 
 
@@ -131,6 +131,7 @@ def load_library():
     L.lbm_adjoint.argtypes = [vp, dp, dp, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_uint64),
                               dp, dp, dp]
     L.lbm_set_adjoint_memory.argtypes = [vp, ctypes.c_int64]
+    L.lbm_set_poisson_graph.argtypes = [vp, ctypes.c_int]
     L.lbm_adjoint_stats.argtypes = [vp, ctypes.POINTER(AdjointStats)]
     _lib = L
     return L
@@ -285,6 +286,10 @@ class InamuroHIP:
     def setUsePoissonDualResidual(self, enabled: bool):
         ci, tol, lim, _ = self._cfg()
         self.setPoisson(ci, tol, lim, enabled)
+
+    def setUsePoissonGraph(self, enabled: bool):
+        """Replay each check-interval segment as a captured HIP graph (default on)."""
+        self._check(load_library().lbm_set_poisson_graph(self._h, int(bool(enabled))))
 
     def set_iteration_schedule(self, iterations):
         """Freeze the Poisson iteration count of the next len(iterations) steps."""

@@ -91,6 +91,11 @@ void InamuroHIP::setEnablePoissonFixedPointShadow(bool enabled)
     apply();
 }
 
+void InamuroHIP::setUsePoissonGraph(bool enabled)
+{
+    check(lbm_set_poisson_graph(ctx, enabled ? 1 : 0), ctx);
+}
+
 lbm_poisson_diag_t InamuroHIP::getLastPoissonDiagnostics() const
 {
     lbm_poisson_diag_t d;

@@ -34,6 +34,9 @@ public:
     void setPoissonIterationLimit(std::uint64_t max_iterations);
     void setUsePoissonDualResidual(bool enabled);
     void setEnablePoissonFixedPointShadow(bool enabled);
+    // Segment replay as a captured HIP graph (GPU/InamuroCUDA.hpp:80); on by
+    // default, results identical either way.
+    void setUsePoissonGraph(bool enabled);
     lbm_poisson_diag_t getLastPoissonDiagnostics() const;
     void downloadHH(std::vector<double>& hh) const;
     // fei, rho, u, v, w, p in the canonical x-fastest layout (each n values).

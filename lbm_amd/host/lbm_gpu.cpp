@@ -2,7 +2,8 @@
 // strict Poisson-contract validation (GPU/main.cpp:18-381).  The time step
 // runs on the MI355X through lbm::Inamuro; flags of the reference's Poisson
 // accelerator variants are parsed and rejected unless left at their default,
-// since this build implements only the strict split contract.
+// since this build implements only the strict split contract.  Graph replay of
+// Poisson segments (--poisson-graph) is always on here.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -43,6 +44,7 @@ struct RunConfig {
     bool write_output = false;
     bool print_roofline = true;
     bool poisson_fixed_point_shadow = false;
+    bool poisson_graph = false;
     bool poisson_dual_residual = false;
     std::string poisson_diagnostics;
     std::string dump_fields;
@@ -114,7 +116,7 @@ RunConfig parse_args(int argc, char** argv)
         else if (arg == "--output-steps") cfg.output_steps = parse_output_steps(value());
         else if (arg == "--write-output") cfg.write_output = true;
         else if (arg == "--poisson-detail") {
-        } else if (arg == "--poisson-graph") accel(true);
+        } else if (arg == "--poisson-graph") cfg.poisson_graph = true;
         else if (arg == "--poisson-deterministic-reductions") {
         } else if (arg == "--poisson-fixed-point-shadow") cfg.poisson_fixed_point_shadow = true;
         else if (arg == "--poisson-dual-residual") {
@@ -151,6 +153,12 @@ RunConfig parse_args(int argc, char** argv)
         throw std::runtime_error(
             "only the strict split+split Poisson contract with unit relaxation is implemented"
             + (cfg.accelerator ? " (" + cfg.accelerator_flag + " is a Poisson accelerator variant)" : std::string()));
+    // GPU/main.cpp:229-238 (graph replay itself is always on here; the flag
+    // combination is rejected as the reference rejects it).
+    if (cfg.poisson_fixed_point_shadow && cfg.poisson_graph)
+        throw std::runtime_error(
+            "fixed-point residual requires GPU split+split, unit relaxation, "
+            "and source-aware/graph/Anderson/two-grid disabled");
     if (!cfg.precision.empty() && cfg.precision != "fp64" && cfg.precision != "mixed")
         throw std::runtime_error("--precision must be fp64 or mixed");
     // Steps and output frequency from params.in lines 3-4, GPU/main.cpp:98-118.

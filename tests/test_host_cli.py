@@ -26,6 +26,8 @@ def run(args, cwd=None):
     (["--poisson-anderson-beta-max", "-1"], "--poisson-anderson-beta-max must be non-negative"),
     (["--poisson", "onepass"], "only the strict split+split Poisson contract"),
     (["--poisson-anderson-m1"], "--poisson-anderson-m1 is a Poisson accelerator variant"),
+    (["--poisson-graph", "--poisson-fixed-point-shadow"],
+     "fixed-point residual requires GPU split+split, unit relaxation, and source-aware/graph/Anderson/two-grid disabled"),
     (["--bogus"], "unknown argument: --bogus"),
     (["--steps"], "missing value for --steps"),
     (["--output-steps", "3,-1"], "--output-steps values must be non-negative"),
