@@ -382,12 +382,13 @@ void poisson_iteration(Ctx& c, int mode)
         lbm::f64::poisson_iter(c.stream, mode, g, D<double>(c.h[c.hcur]), D<double>(c.h[1 - c.hcur]),
                                c.p[c.pc], D<double>(c.tau), D<double>(c.src), c.dst);
     } else {
-        // Decodes with base pb, writes this iteration's bases into the other
-        // buffer; the cell pressure of the non-streamed modes comes from p.
+        // Decodes with base pb.  The first iteration of a segment (equilibrium,
+        // explicit or gauged entry) writes new bases into the other buffer;
+        // streamed iterations encode against base pb itself.
         lbm::mx::poisson_iter(c.stream, mode, g, D<float>(c.h[c.hcur]), D<float>(c.h[1 - c.hcur]),
                               c.base[c.pb], c.p[c.pc], c.base[1 - c.pb], D<float>(c.tau),
                               D<float>(c.src), c.dst);
-        c.pb = 1 - c.pb;
+        if (mode != lbm::H_STREAMED) c.pb = 1 - c.pb;
     }
     c.hcur = 1 - c.hcur;
 }
