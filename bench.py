@@ -44,6 +44,11 @@ PHYSICS = dict(rho_L=1000.0, rho_G=1.0, mu_L=0.008, mu_G=8e-5, tauf=1.0, taug=1.
                k_g=1e-4, T=0.035, a=1.0, b=6.7, fei_max=0.093, fei_min=0.016, fei_L=0.093,
                fei_G=0.016)
 
+# Replay Poisson segments as captured HIP graphs (results identical either
+# way); --no-graphs launches them one by one, for profilers that cannot trace
+# graph capture.
+GRAPHS = True
+
 WORKLOADS = {
     "static_droplet_256": dict(L=256, R=64),
     "static_droplet_64": dict(L=64, R=16),
@@ -54,6 +59,7 @@ WORKLOADS = {
 def make_sim(hip, L, R, precision):
     P = hip.SolverParams(lx=L, ly=L, lz=L, DD=2.0 * R, **PHYSICS)
     sim = hip.InamuroHIP(P, precision={"fp64": hip.FP64, "mixed": hip.MIXED}[precision])
+    sim.setUsePoissonGraph(GRAPHS)
     sim.reset_fields()
     sim.place_droplet(L / 2.0, L / 2.0, L / 2.0, profile="tanh", width=2.0)
     return sim, P
@@ -146,6 +152,7 @@ def run_adjoint(hip, steps, checkpoint):
     phi0 = ellipsoid_phi(L, ADJ["axes"])
     target = ellipsoid_phi(L, (ADJ["R_target"],) * 3)
     sim = hip.InamuroHIP(P)
+    sim.setUsePoissonGraph(GRAPHS)
     sim.initFromCPU(fei=phi0, rho=sim.initial_rho(phi0), u=0.0, v=0.0, w=0.0, p=0.0)
     sim.synchronize()
     t0 = time.perf_counter()
@@ -213,6 +220,8 @@ def main(argv=None):
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint leg and print its record")
     ap.add_argument("--adjoint-steps", type=int, default=8)
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch Poisson iterations one by one instead of replaying graphs")
     ap.add_argument("--adjoint-checkpoint", type=int, default=4)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one short solve per precision (for rocprofv3 --pmc runs) and exit")
@@ -223,6 +232,9 @@ def main(argv=None):
         return 2
 
     from lbm_amd import hip
+
+    global GRAPHS
+    GRAPHS = not args.no_graphs
 
     if args.adjoint_only:
         print(json.dumps(run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint)), flush=True)
