@@ -50,23 +50,45 @@ PHYSICS = dict(rho_L=1000.0, rho_G=1.0, mu_L=0.008, mu_G=8e-5, tauf=1.0, taug=1.
 GRAPHS = True
 
 WORKLOADS = {
-    "static_droplet_256": dict(L=256, R=64),
-    "static_droplet_64": dict(L=64, R=16),
-    "static_droplet_128": dict(L=128, R=32),
+    "static_droplet_256": dict(shape=(256, 256, 256), R=64),
+    "static_droplet_64": dict(shape=(64, 64, 64), R=16),
+    "static_droplet_128": dict(shape=(128, 128, 128), R=32),
+    # two R=24 droplets at x=80 and x=176 moving at -/+0.02 towards each other,
+    # radii perturbed by seeded real spherical harmonics 2<=l<=8 (RMS 0.5)
+    "head_on_collision": dict(shape=(128, 128, 256), R=24, head_on=True),
 }
 
 
-def make_sim(hip, L, R, precision):
-    P = hip.SolverParams(lx=L, ly=L, lz=L, DD=2.0 * R, **PHYSICS)
+def workload_params(hip, wl):
+    lz, ly, lx = wl["shape"]
+    return hip.SolverParams(lx=lx, ly=ly, lz=lz, DD=2.0 * wl["R"], **PHYSICS)
+
+
+def initial_fields(wl, P):
+    """Host fields of the workload (None: the library's place_droplet)."""
+    if not wl.get("head_on"):
+        return None
+    from lbm_amd import configs
+
+    return configs.head_on_collision(P, shape=wl["shape"], radius=wl["R"])
+
+
+def make_sim(hip, wl, precision):
+    P = workload_params(hip, wl)
     sim = hip.InamuroHIP(P, precision={"fp64": hip.FP64, "mixed": hip.MIXED}[precision])
     sim.setUsePoissonGraph(GRAPHS)
-    sim.reset_fields()
-    sim.place_droplet(L / 2.0, L / 2.0, L / 2.0, profile="tanh", width=2.0)
+    fields = initial_fields(wl, P)
+    if fields is None:
+        lz, ly, lx = wl["shape"]
+        sim.reset_fields()
+        sim.place_droplet(lx / 2.0, ly / 2.0, lz / 2.0, profile="tanh", width=2.0)
+    else:
+        sim.upload_fields(fields)
     return sim, P
 
 
-def run_precision(hip, L, R, precision, steps, warmup):
-    sim, P = make_sim(hip, L, R, precision)
+def run_precision(hip, wl, precision, steps, warmup):
+    sim, P = make_sim(hip, wl, precision)
     sim.set_timing(True)
     for _ in range(warmup):
         sim.performTimeStepGPU(1)
@@ -80,7 +102,7 @@ def run_precision(hip, L, R, precision, steps, warmup):
     sim.synchronize()
     t1 = time.perf_counter()
     pf = sim.perf()
-    cells = L * L * L
+    cells = P.lx * P.ly * P.lz
     wall = t1 - t0
     avg_iter_ms = pf.poisson_ms / max(pf.poisson_iter_launches, 1)
     bpi = POISSON_BYTES[precision] * cells
@@ -127,30 +149,16 @@ def field_errors(mixed, fp64):
 ADJ = dict(L=96, axes=(20.0, 16.0, 16.0), R_target=17.2)
 
 
-def ellipsoid_phi(L, axes, width=2.0):
-    """phi0 = mean + (phi_L - phi_G)/2 tanh(d / width), with d the
-    first-order signed distance to the ellipsoid, (1 - s) / |grad s|."""
-    import numpy as np
-
-    c = L / 2.0
-    z, y, x = np.meshgrid(*(np.arange(L, dtype=np.float64) - c for _ in range(3)), indexing="ij")
-    a, b, cc = axes
-    s = np.sqrt((x / a) ** 2 + (y / b) ** 2 + (z / cc) ** 2)
-    gs = np.sqrt((x / a ** 2) ** 2 + (y / b ** 2) ** 2 + (z / cc ** 2) ** 2)
-    with np.errstate(divide="ignore", invalid="ignore"):
-        d = np.where(s > 0, (1.0 - s) * s / np.where(gs > 0, gs, 1.0), min(axes))
-    P = PHYSICS
-    return 0.5 * (P["fei_L"] + P["fei_G"]) + 0.5 * (P["fei_L"] - P["fei_G"]) * np.tanh(d / width)
-
-
 def run_adjoint(hip, steps, checkpoint):
     """Plain forward of `steps` steps, then lbm_adjoint of the same map with a
     checkpoint every `checkpoint` steps; cost ratio = reverse sweep (replays
     and recomputation included) over the plain forward."""
     L = ADJ["L"]
     P = hip.SolverParams(lx=L, ly=L, lz=L, DD=2.0 * ADJ["R_target"], **PHYSICS)
-    phi0 = ellipsoid_phi(L, ADJ["axes"])
-    target = ellipsoid_phi(L, (ADJ["R_target"],) * 3)
+    from lbm_amd import configs
+
+    phi0 = configs.ellipsoid_phi(L, ADJ["axes"], P)
+    target = configs.ellipsoid_phi(L, (ADJ["R_target"],) * 3, P)
     sim = hip.InamuroHIP(P)
     sim.setUsePoissonGraph(GRAPHS)
     sim.initFromCPU(fei=phi0, rho=sim.initial_rho(phi0), u=0.0, v=0.0, w=0.0, p=0.0)
@@ -176,15 +184,22 @@ def run_adjoint(hip, steps, checkpoint):
     )
 
 
-def cpu_baseline(L, R, n_it_per_step):
-    """Single-thread CPU oracle on the same 256^3 state: one non-Poisson step
+def cpu_baseline(hip, wl, workload, n_it_per_step):
+    """Single-thread CPU oracle on the same initial state: one non-Poisson step
     part plus two Poisson iterations, extrapolated with the GPU's measured
     iterations per step."""
     from oracle import pyoracle as po
 
-    P = po.Params(lx=L, ly=L, lz=L, DD=2.0 * R, **PHYSICS)
+    lz, ly, lx = wl["shape"]
+    P = po.Params(lx=lx, ly=ly, lz=lz, DD=2.0 * wl["R"], **PHYSICS)
     o = po.Oracle(P, threads=1)
-    o.place_droplet(L / 2.0, L / 2.0, L / 2.0, profile="tanh", width=2.0)
+    fields = initial_fields(wl, P)
+    if fields is None:
+        o.place_droplet(lx / 2.0, ly / 2.0, lz / 2.0, profile="tanh", width=2.0)
+    else:
+        for k, v in fields.items():
+            o.set(k, v)
+    cells = lx * ly * lz
     t0 = time.perf_counter()
     for st in ("collision", "stream_fg", "bc_fg", "macro"):
         o.stage(st)
@@ -200,8 +215,8 @@ def cpu_baseline(L, R, n_it_per_step):
     t_step = t_np + n_it_per_step * t_it
     o.close()
     return dict(
-        value=L ** 3 / t_step / 1e6, unit="MLUPS", cores=1, kind="port",
-        sample=(f"{L}^3 static-droplet state: one collision+stream+boundary+macro+correction pass "
+        value=cells / t_step / 1e6, unit="MLUPS", cores=1, kind="port",
+        sample=(f"{workload} initial state ({lx}x{ly}x{lz}): one collision+stream+boundary+macro+correction pass "
                 f"({t_np:.2f} s) and {k} Poisson iterations ({t_it:.3f} s each) timed on one host core, "
                 f"extrapolated to {n_it_per_step:.0f} iterations per step (the GPU's measured mean)"),
         host_cpu=platform.processor() or platform.machine(), nproc=os.cpu_count(),
@@ -240,19 +255,19 @@ def main(argv=None):
         print(json.dumps(run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint)), flush=True)
         return 0
     wl = WORKLOADS[args.workload]
-    L, R = wl["L"], wl["R"]
+    lz, ly, lx = wl["shape"]
     precs = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     if args.traffic_probe:
         # One step from the cold start per precision: the first solve runs 100
         # Poisson iterations (one check), enough dispatches for counter runs.
         for p in precs:
-            sim, _ = make_sim(hip, L, R, p)
+            sim, _ = make_sim(hip, wl, p)
             sim.performTimeStepGPU(1)
             sim.synchronize()
             print(f"traffic probe {p}: {sim.getLastPoissonDiagnostics().iterations} iterations", flush=True)
             sim.close()
         return 0
-    results = {p: run_precision(hip, L, R, p, args.steps, args.warmup) for p in precs}
+    results = {p: run_precision(hip, wl, p, args.steps, args.warmup) for p in precs}
     head = results["mixed"] if "mixed" in results else results["fp64"]
     n_it_mean = sum(head["iterations"]) / len(head["iterations"])
     line = {
@@ -269,9 +284,12 @@ def main(argv=None):
         "dtype": "f64" if head["precision"] == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"{args.workload}: {L}^3 D3Q15 grid, tanh droplet R={R} at the centre, "
-                        "density ratio 1000, Poisson tol 1e-3 / check 100, steps after a cold start",
-            "grid": [L, L, L], "radius": R, "precision": head["precision"],
+            "workload": (f"{args.workload}: {lx}x{ly}x{lz} D3Q15 grid, "
+                         + ("two tanh droplets R=24 at x=80/176, velocities +/-0.02, radii perturbed by seeded "
+                            "spherical harmonics 2<=l<=8 (RMS 0.5)" if wl.get("head_on") else
+                            f"tanh droplet R={wl['R']} at the centre")
+                         + ", density ratio 1000, Poisson tol 1e-3 / check 100, steps after a cold start"),
+            "grid": [lx, ly, lz], "radius": wl["R"], "precision": head["precision"],
             "poisson_iterations_per_step": head["iterations"],
         },
         "roofline": {
@@ -312,7 +330,7 @@ def main(argv=None):
     if not args.no_adjoint:
         line["adjoint"] = run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint)
     if not args.no_cpu_baseline:
-        line["cpu_baseline"] = cpu_baseline(L, R, n_it_mean)
+        line["cpu_baseline"] = cpu_baseline(hip, wl, args.workload, n_it_mean)
     print(json.dumps(line), flush=True)
     return 0
 
