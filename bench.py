@@ -149,6 +149,42 @@ def field_errors(mixed, fp64):
 ADJ = dict(L=96, axes=(20.0, 16.0, 16.0), R_target=17.2)
 
 
+def adjoint_fd(hip, steps, checkpoint, directions, h=1e-6):
+    """The adjoint leg's gradient against central differences of the same
+    frozen-count map at full size: `directions` seeded unit-Gaussian
+    directions over phi0 (step h) and the k_g direction (step h)."""
+    import dataclasses
+
+    import numpy as np
+    from lbm_amd import configs
+
+    L = ADJ["L"]
+    P = hip.SolverParams(lx=L, ly=L, lz=L, DD=2.0 * ADJ["R_target"], **PHYSICS)
+    phi0 = configs.ellipsoid_phi(L, ADJ["axes"], P)
+    target = configs.ellipsoid_phi(L, (ADJ["R_target"],) * 3, P)
+    sim = hip.InamuroHIP(P)
+    sim.setUsePoissonGraph(GRAPHS)
+    J, g, gk = sim.adjoint(phi0, target, steps, checkpoint_interval=checkpoint)
+    its = sim.iteration_history()[-steps:]
+    rng = np.random.default_rng(3)
+    rows = []
+    for i in range(directions):
+        d = rng.standard_normal(phi0.shape)
+        fd = (sim.objective(phi0 + h * d, target, its) - sim.objective(phi0 - h * d, target, its)) / (2 * h)
+        ad = float((g * d).sum())
+        rows.append(dict(fd=fd, adjoint=ad, rel=abs(fd - ad) / abs(fd)))
+        print(f"adjoint fd direction {i}: fd {fd:.10e} adjoint {ad:.10e} rel {rows[-1]['rel']:.2e}",
+              file=sys.stderr, flush=True)
+    sim.close()
+    sp = hip.InamuroHIP(dataclasses.replace(P, k_g=P.k_g + h))
+    sm = hip.InamuroHIP(dataclasses.replace(P, k_g=P.k_g - h))
+    fdk = (sp.objective(phi0, target, its) - sm.objective(phi0, target, its)) / (2 * h)
+    return dict(workload=f"ellipsoid_96, {steps} steps from rest, checkpoint every {checkpoint}", h=h,
+                poisson_iterations_per_step=its, J=J,
+                phi0_directions=rows, max_rel_phi0=max(r["rel"] for r in rows),
+                kg=dict(fd=fdk, adjoint=gk, rel=abs(fdk - gk) / abs(fdk)))
+
+
 def run_adjoint(hip, steps, checkpoint):
     """Plain forward of `steps` steps, then lbm_adjoint of the same map with a
     checkpoint every `checkpoint` steps; cost ratio = reverse sweep (replays
@@ -235,6 +271,8 @@ def main(argv=None):
     ap.add_argument("--adjoint-only", action="store_true",
                     help="run only the adjoint leg and print its record")
     ap.add_argument("--adjoint-steps", type=int, default=8)
+    ap.add_argument("--adjoint-fd", type=int, default=0, metavar="N",
+                    help="with --adjoint-only: also check N directions against central differences")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch Poisson iterations one by one instead of replaying graphs")
     ap.add_argument("--adjoint-checkpoint", type=int, default=4)
@@ -252,7 +290,11 @@ def main(argv=None):
     GRAPHS = not args.no_graphs
 
     if args.adjoint_only:
-        print(json.dumps(run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint)), flush=True)
+        rec = run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint)
+        if args.adjoint_fd:
+            rec["finite_differences"] = adjoint_fd(hip, args.adjoint_steps, args.adjoint_checkpoint,
+                                                   args.adjoint_fd)
+        print(json.dumps(rec), flush=True)
         return 0
     wl = WORKLOADS[args.workload]
     lz, ly, lx = wl["shape"]

@@ -1,0 +1,70 @@
+"""Writes the long config-0 fixture from the oracle alone:
+
+    python tests/golden/make_oracle_long.py [steps]
+
+The 64^3 config-0 droplet (tanh R=16, density ratio 1000) for `steps` steps
+(default 20).  The compiled reference takes over an hour for the first three
+steps of this case, so only those three are reference-generated
+(config0_64.json, where the oracle is asserted bit-identical to it); this
+fixture continues the same trajectory with the oracle, whose first three
+steps it re-checks against config0_64.json.  Same schema as make_golden.py's
+fixtures, with "generated_by": "oracle".
+"""
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import pyoracle as po  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def main(argv):
+    steps = int(argv[1]) if len(argv) > 1 else 20
+    base = json.load(open(os.path.join(OUT, "config0_64.json")))
+    P = po.Params(**base["params"])
+    o = po.Oracle(P, threads=1)
+    cx, cy, cz = base["init"]["center"]
+    o.place_droplet(cx, cy, cz, profile=base["init"]["kind"], width=base["init"]["width"])
+    init_sha = {k: sha(v) for k, v in o.fields().items()}
+    assert init_sha == base["init_sha256"]
+    iters, step_sha = [], []
+    t0 = time.time()
+    for s in range(steps):
+        iters += o.step(1)
+        step_sha.append({k: sha(v) for k, v in o.fields().items()})
+        if s < base["steps"]:
+            assert iters[s] == base["iterations"][s] and step_sha[s] == base["step_sha256"][s], s
+        print(f"step {s + 1}: {iters[-1]} iterations, {time.time() - t0:.0f} s", flush=True)
+    final = o.fields()
+    n = P.lx * P.ly * P.lz
+    idx = np.sort(np.random.default_rng(20261203).choice(n, size=4096, replace=False))
+    rec = dict(case=f"config0_64_oracle{steps}", generated_by="oracle",
+               params=base["params"], init=base["init"], steps=steps, iterations=iters,
+               oracle_seconds=time.time() - t0, fields={}, init_sha256=init_sha, step_sha256=step_sha,
+               sample_index_seed=20261203)
+    samples = {"index": idx.astype(np.int64)}
+    for k in po.FIELD_NAMES:
+        a = final[k].reshape(-1)
+        rec["fields"][k] = dict(sha256=sha(final[k]), sum=float(a.sum()), sum_abs=float(np.abs(a).sum()),
+                                max_abs=float(np.abs(a).max()), min=float(a.min()), max=float(a.max()))
+        samples[k] = a[idx].copy()
+    name = f"config0_64_oracle{steps}"
+    json.dump(rec, open(os.path.join(OUT, name + ".json"), "w"), indent=1)
+    np.savez_compressed(os.path.join(OUT, name + ".npz"), **samples)
+    print(f"wrote {name}: iterations {iters}", flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv)
