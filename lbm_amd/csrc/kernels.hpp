@@ -81,7 +81,8 @@ void macro(hipStream_t s, int pull, const Grid& g, const Phys& ph, const double*
            const DevState* st, double* fei, double* rho, double* u, double* v, double* w,
            double* tau, double* src);
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, double* hout,
-                  const double* p, const double* tau, const double* src, const DevState* st);
+                  const double* p, const double* tau, const double* src, const DevState* st,
+                  bool nontemporal);
 void check(hipStream_t s, const Grid& g, const Phys& ph, double* p, double* p_prev,
            const double* hin, const double* tau, const double* src, int dual, double tol,
            double* partials, DevState* st);

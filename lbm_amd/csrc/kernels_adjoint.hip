@@ -71,12 +71,13 @@ __device__ __forceinline__ void gather_stream_slip_T(int x, int y, int z, const 
 }
 
 // The 15 streamed values of cell c seen through the x-slip rule (the forward
-// Poisson kernels' load_streamed).
+// Poisson kernels' load_streamed).  A recorded iterate is read once, so the
+// load carries a non-temporal hint and leaves the cache to the adjoint state.
 __device__ __forceinline__ void load_view(const double* __restrict__ hin, int64_t c, int x,
                                           const Grid& g, double (&hv)[Q])
 {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) hv[q] = hin[q * g.n + c];
+    for (int q = 0; q < Q; ++q) hv[q] = __builtin_nontemporal_load(&hin[q * g.n + c]);
     if (x == 0) {
 #pragma unroll
         for (int r = 0; r < 5; ++r) hv[SLIP_POS[r]] = hv[SLIP_NEG[r]];
@@ -243,7 +244,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
             gather_stream_slip_T(x, y, z, g, [&](int, int64_t d) { return lpf[d] - ls; }, lh);
         } else {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) lh[q] = Bin[q * g.n + c];
+            for (int q = 0; q < Q; ++q) lh[q] = __builtin_nontemporal_load(&Bin[q * g.n + c]);
             if (SRC == SRC_LAZY) {
                 const double ls = as->lam_shift;
 #pragma unroll

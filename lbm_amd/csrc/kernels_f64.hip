@@ -280,11 +280,12 @@ __global__ __launch_bounds__(BLOCK) void k_macro(
 // Loads the 15 streamed populations of cell c and applies the x-slip rule
 // (CPU/Inamuro.cpp:589-623): the pressure populations are stored streamed
 // but not yet bounced, so every reader sees them through this view.
+template <int NT = 0>
 __device__ __forceinline__ void load_streamed(const double* __restrict__ hin, int64_t c, int x,
                                               const Grid& g, double (&hv)[Q])
 {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) hv[q] = hin[q * g.n + c];
+    for (int q = 0; q < Q; ++q) hv[q] = NT ? __builtin_nontemporal_load(&hin[q * g.n + c]) : hin[q * g.n + c];
     if (x == 0) {
 #pragma unroll
         for (int r = 0; r < 5; ++r) hv[SLIP_POS[r]] = hv[SLIP_NEG[r]];
@@ -302,7 +303,10 @@ __device__ __forceinline__ void load_streamed(const double* __restrict__ hin, in
 // Every value is the one the reference computes; only the place where the
 // slip rule and the pressure sum are evaluated moves, so all loads are the
 // cell's own (aligned, coalesced) and p is written only at checks.
-template <int MODE>
+// NT: the streamed h is loaded with a non-temporal hint (it is dead after
+// this read), for grids whose two h buffers do not fit the last-level cache
+// and for the adjoint's recomputation.
+template <int MODE, int NT>
 __global__ __launch_bounds__(BLOCK) void k_poisson_push(
     Grid g, const double* __restrict__ hin, double* __restrict__ hout,
     const double* __restrict__ pbuf, const double* __restrict__ tau,
@@ -323,7 +327,7 @@ __global__ __launch_bounds__(BLOCK) void k_poisson_push(
         for (int q = 0; q < Q; ++q) hv[q] = hin[q * g.n + c];
         p = pbuf[c];
     } else {
-        load_streamed(hin, c, x, g, hv);
+        load_streamed<NT>(hin, c, x, g, hv);
         if (MODE == H_GAUGED) {
             const double shift = st->shift;
 #pragma unroll
@@ -672,23 +676,24 @@ void macro(hipStream_t s, int pull, const Grid& g, const Phys& ph, const double*
 }
 
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, double* hout,
-                  const double* p, const double* tau, const double* src, const DevState* st)
+                  const double* p, const double* tau, const double* src, const DevState* st,
+                  bool nontemporal)
 {
     const int nb = grid_blocks(g.n);
+#define LBM_PUSH(M)                                                                          \
+    do {                                                                                     \
+        if (nontemporal)                                                                     \
+            k_poisson_push<M, 1><<<nb, BLOCK, 0, s>>>(g, hin, hout, p, tau, src, st);        \
+        else                                                                                 \
+            k_poisson_push<M, 0><<<nb, BLOCK, 0, s>>>(g, hin, hout, p, tau, src, st);        \
+    } while (0)
     switch (mode) {
-    case H_EQUILIBRIUM:
-        k_poisson_push<H_EQUILIBRIUM><<<nb, BLOCK, 0, s>>>(g, hin, hout, p, tau, src, st);
-        break;
-    case H_GAUGED:
-        k_poisson_push<H_GAUGED><<<nb, BLOCK, 0, s>>>(g, hin, hout, p, tau, src, st);
-        break;
-    case H_EXPLICIT:
-        k_poisson_push<H_EXPLICIT><<<nb, BLOCK, 0, s>>>(g, hin, hout, p, tau, src, st);
-        break;
-    default:
-        k_poisson_push<H_STREAMED><<<nb, BLOCK, 0, s>>>(g, hin, hout, p, tau, src, st);
-        break;
+    case H_EQUILIBRIUM: LBM_PUSH(H_EQUILIBRIUM); break;
+    case H_GAUGED: LBM_PUSH(H_GAUGED); break;
+    case H_EXPLICIT: LBM_PUSH(H_EXPLICIT); break;
+    default: LBM_PUSH(H_STREAMED); break;
     }
+#undef LBM_PUSH
 }
 
 void check(hipStream_t s, const Grid& g, const Phys& ph, double* p, double* p_prev,
