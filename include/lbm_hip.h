@@ -176,6 +176,18 @@ int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance,
  * either way; only launch overhead changes.  On by default here. */
 int lbm_set_poisson_graph(void* ctx, int enabled);
 
+/* No counterpart in the reference (extensions for the falling-droplet
+ * workload), both off by default:
+ * lbm_set_z_walls: no-slip walls at z = 0 and z = lz-1 -- halfway bounce-back
+ *   of f, g and h in streaming, zero normal gradient in the stencils and in
+ *   correct_uvw.  Not combined with the dual residual.
+ * lbm_set_gravity: gravitational acceleration gz along z (negative = down),
+ *   applied to the predicted velocity as the buoyant acceleration
+ *   gz (1 - rho_G/rho).
+ * The adjoint covers the reference configuration only (neither set). */
+int lbm_set_z_walls(void* ctx, int enabled);
+int lbm_set_gravity(void* ctx, double gz);
+
 /* Frozen Poisson iteration counts (no counterpart in the reference): step i
  * after this call runs exactly iterations[i] iterations, gauging at every
  * check, whatever the residual; later steps converge as usual.  This makes the

@@ -398,6 +398,14 @@ void poisson_iteration(Ctx& c, int mode)
     c.hcur = 1 - c.hcur;
 }
 
+// Captured segments hold the launch arguments by value (the grid and its
+// boundary mode included); settings that change them drop the cache.
+void clear_graphs(Ctx& c)
+{
+    for (auto& gph : c.graphs) (void)hipGraphExecDestroy(gph.exec);
+    c.graphs.clear();
+}
+
 // check_interval iterations, the first in `mode`, the rest H_STREAMED:
 // eagerly, or by replaying a captured graph of exactly these launches
 // (buildPoissonGraphSegment, GPU/InamuroCUDA.cu:1482-1534).  Kernel nodes only.
@@ -958,6 +966,8 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
                  double* J, double* grad_phi0, double* grad_kg)
 {
     if (c.mixed) throw ApiError{LBM_ERR_ARG, "the adjoint runs on an FP64 context"};
+    if (c.g.zwall || c.ph.gz != 0.0)
+        throw ApiError{LBM_ERR_ARG, "the adjoint covers the reference boundaries without gravity"};
     if (!phi0 || !target || !J || T < 1 || K < 1)
         throw ApiError{LBM_ERR_ARG, "adjoint needs phi0, target, objective, steps >= 1 and checkpoint interval >= 1"};
     const uint64_t ci = static_cast<uint64_t>(c.check_interval);
@@ -1300,7 +1310,30 @@ int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance, uint64_
         c.iter_limit = iteration_limit;
         if (dual_residual < 0 || dual_residual > 2)
             throw ApiError{LBM_ERR_ARG, "dual_residual must be 0 (off), 1 (dual exit) or 2 (shadow)"};
+        if (dual_residual != 0 && c.g.zwall)
+            throw ApiError{LBM_ERR_ARG, "the dual residual covers the reference boundaries only (no z walls)"};
         c.dual = dual_residual;
+        clear_graphs(c);
+    });
+}
+
+int lbm_set_z_walls(void* ctx, int enabled)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        if (enabled && c.P.lz < 2) throw ApiError{LBM_ERR_ARG, "z walls need lz >= 2"};
+        if (enabled && c.dual != 0)
+            throw ApiError{LBM_ERR_ARG, "the dual residual covers the reference boundaries only (no z walls)"};
+        c.g.zwall = enabled ? 1 : 0;
+        clear_graphs(c);
+    });
+}
+
+int lbm_set_gravity(void* ctx, double gz)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        if (!std::isfinite(gz)) throw ApiError{LBM_ERR_ARG, "gravity must be finite"};
+        c.ph.gz = gz;
+        c.phf.gz = static_cast<float>(gz);
     });
 }
 

@@ -32,13 +32,16 @@ constexpr double FW[Q] = {-7.0 / 3.0,
 // Specular x-wall pairs (+x direction, -x direction), CPU/Inamuro.cpp:592-593.
 constexpr int SLIP_POS[5] = {1, 7, 9, 10, 12};
 constexpr int SLIP_NEG[5] = {4, 8, 14, 13, 11};
+// Opposite direction, c_opp = -c (bounce-back at the optional z walls).
+constexpr int OPP[Q] = {0, 4, 5, 6, 1, 2, 3, 11, 12, 13, 14, 7, 8, 9, 10};
 
 constexpr int BLOCK = 256;       // threads per block: 4 wave64s
 constexpr int RED_BLOCKS = 1024; // fixed grid of the deterministic reductions
 
 struct Grid {
     int lx, ly, lz;
-    int64_t n;  // lx * ly * lz
+    int64_t n;      // lx * ly * lz
+    int zwall = 0;  // 1: no-slip walls at z = 0 and lz-1 (extension); 0: periodic z
 };
 
 // Cell coordinates of linear index c = (z*ly + y)*lx + x.
@@ -69,6 +72,29 @@ __device__ __forceinline__ int mirror_x(int x, int d, int lx)
     r = r >= lx ? lx - 2 : r;
     r = r < 0 ? 1 : r;
     return r;
+}
+
+// z neighbour of the stencils: periodic, or (z walls) the cell itself beyond
+// a wall, i.e. zero normal gradient about a wall half a cell outside.
+__device__ __forceinline__ int z_stencil(int z, int d, const Grid& g)
+{
+    if (g.zwall) {
+        const int r = z + d;
+        return r < 0 ? 0 : (r >= g.lz ? g.lz - 1 : r);
+    }
+    return wrap(z, d, g.lz);
+}
+
+// Halfway bounce-back at the z walls: the population arriving at z in
+// direction q would come from beyond a wall (pull side) ...
+__device__ __forceinline__ bool z_bounced_in(int q, int z, const Grid& g)
+{
+    return g.zwall && ((CZ[q] == 1 && z == 0) || (CZ[q] == -1 && z == g.lz - 1));
+}
+// ... or the population leaving z in direction q would cross a wall (push side).
+__device__ __forceinline__ bool z_bounced_out(int q, int z, const Grid& g)
+{
+    return g.zwall && ((CZ[q] == 1 && z == g.lz - 1) || (CZ[q] == -1 && z == 0));
 }
 
 __device__ __forceinline__ int64_t lin(int x, int y, int z, const Grid& g)

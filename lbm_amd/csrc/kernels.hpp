@@ -17,6 +17,7 @@ struct Phys {
     double tmp1;     // (mu_L - mu_G) / (rho_L - rho_G), CPU/Inamuro.cpp:137
     double fei_avg;  // (fei_L + fei_G) / 2.0, CPU/Inamuro.cpp:296
     double cell_count;
+    double gz = 0.0;  // extension: gravitational acceleration along z
 };
 
 // Device-resident state of one Poisson solve; the host reads it back once per
@@ -149,6 +150,7 @@ namespace mx {
 struct PhysF {
     float rho_L, rho_G, mu_G, tauf_inv, taug_inv, k_f, k_g, T, a, b, fei_L, fei_G;
     float tmp1, fei_avg, visc;  // visc = 1.5 * (taug - 0.5) * 2.0
+    float gz = 0.f;             // extension: gravitational acceleration along z
 };
 
 void collide(hipStream_t s, int mode, const Grid& g, const PhysF& ph, const float* fei,

@@ -31,7 +31,10 @@ __device__ __forceinline__ void pull_slip(const double* __restrict__ in, int x, 
         const int sx = CX[q] == 1 ? xm : (CX[q] == -1 ? xp : x);
         const int sy = CY[q] == 1 ? ym : (CY[q] == -1 ? yp : y);
         const int sz = CZ[q] == 1 ? zm : (CZ[q] == -1 ? zp : z);
-        o[q] = in[q * g.n + lin(sx, sy, sz, g)];
+        if (z_bounced_in(q, z, g))
+            o[q] = in[OPP[q] * g.n + lin(x, y, z, g)];  // bounce-back at a z wall
+        else
+            o[q] = in[q * g.n + lin(sx, sy, sz, g)];
     }
     if (x == 0) {
 #pragma unroll
@@ -98,7 +101,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
         int64_t idx[Q];
 #pragma unroll
         for (int k = 0; k < Q; ++k)
-            idx[k] = lin(mirror_x(x, CX[k], g.lx), wrap(y, CY[k], g.ly), wrap(z, CZ[k], g.lz), g);
+            idx[k] = lin(mirror_x(x, CX[k], g.lx), wrap(y, CY[k], g.ly), z_stencil(z, CZ[k], g), g);
 
         double nb[Q];
         double u_x, u_y, u_z, v_x, v_y, v_z, w_x, w_y, w_z;
@@ -170,6 +173,12 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
                 fo = fv[i] - (fv[i] - fequ) / ph.tauf;
                 go = gv[i] - (gv[i] - gequ) / ph.taug +
                      3.0 * EW[i] / rhoc * mu * (UC[i] * u_lap + VC[i] * v_lap + WC[i] * w_lap);
+            }
+            if (CZ[i] != 0 && ph.gz != 0.0) {
+                // extension: buoyant acceleration gz (1 - rho_G/rho) on the
+                // predicted velocity
+                const double az = ph.gz * (1.0 - ph.rho_G / rhoc);
+                go += 3.0 * EW[i] * WC[i] * az;
             }
             fout[i * g.n + c] = fo;
             gout[i * g.n + c] = go;
@@ -351,7 +360,10 @@ __global__ __launch_bounds__(BLOCK) void k_poisson_push(
         const int dx = CX[q] == 1 ? xp : (CX[q] == -1 ? xm : x);
         const int dy = CY[q] == 1 ? yp : (CY[q] == -1 ? ym : y);
         const int dz = CZ[q] == 1 ? zp : (CZ[q] == -1 ? zm : z);
-        hout[q * g.n + lin(dx, dy, dz, g)] = o;
+        if (z_bounced_out(q, z, g))
+            hout[OPP[q] * g.n + c] = o;  // bounce-back at a z wall
+        else
+            hout[q * g.n + lin(dx, dy, dz, g)] = o;
     }
 }
 
@@ -581,8 +593,8 @@ __global__ __launch_bounds__(BLOCK) void k_correct(Grid g, const double* __restr
     const int x_w = (x == 0) ? 1 : x - 1;
     const int y_n = (y == g.ly - 1) ? 0 : y + 1;
     const int y_s = (y == 0) ? g.ly - 1 : y - 1;
-    const int z_n = (z == g.lz - 1) ? 0 : z + 1;
-    const int z_s = (z == 0) ? g.lz - 1 : z - 1;
+    const int z_n = (z == g.lz - 1) ? (g.zwall ? g.lz - 1 : 0) : z + 1;
+    const int z_s = (z == 0) ? (g.zwall ? 0 : g.lz - 1) : z - 1;
     const double inv_2rho = 1.0 / (2.0 * rho[c]);
     u[c] -= (p[lin(x_e, y, z, g)] - p[lin(x_w, y, z, g)]) * inv_2rho;
     v[c] -= (p[lin(x, y_n, z, g)] - p[lin(x, y_s, z, g)]) * inv_2rho;

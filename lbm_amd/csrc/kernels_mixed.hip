@@ -87,7 +87,10 @@ __device__ __forceinline__ void pull_slip(const float* __restrict__ in, int x, i
         const int sx = CX[q] == 1 ? xm : (CX[q] == -1 ? xp : x);
         const int sy = CY[q] == 1 ? ym : (CY[q] == -1 ? yp : y);
         const int sz = CZ[q] == 1 ? zm : (CZ[q] == -1 ? zp : z);
-        o[q] = in[q * g.n + lin(sx, sy, sz, g)];
+        if (z_bounced_in(q, z, g))
+            o[q] = in[OPP[q] * g.n + lin(x, y, z, g)];  // bounce-back at a z wall
+        else
+            o[q] = in[q * g.n + lin(sx, sy, sz, g)];
     }
     slip(o, x, g.lx);
 }
@@ -182,7 +185,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
         int idx[Q];
 #pragma unroll
         for (int k = 0; k < Q; ++k)
-            idx[k] = static_cast<int>(lin(mirror_x(x, CX[k], g.lx), wrap(y, CY[k], g.ly), wrap(z, CZ[k], g.lz), g));
+            idx[k] = static_cast<int>(lin(mirror_x(x, CX[k], g.lx), wrap(y, CY[k], g.ly), z_stencil(z, CZ[k], g), g));
         const Stencil U = stencil(u, idx);
         const Stencil V = stencil(v, idx);
         const Stencil W = stencil(w, idx);
@@ -219,6 +222,14 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
         LBM_MX_DIR(5) LBM_MX_DIR(6) LBM_MX_DIR(7) LBM_MX_DIR(8) LBM_MX_DIR(9)
         LBM_MX_DIR(10) LBM_MX_DIR(11) LBM_MX_DIR(12) LBM_MX_DIR(13) LBM_MX_DIR(14)
 #undef LBM_MX_DIR
+        if (ph.gz != 0.f) {
+            // extension: buoyant acceleration gz (1 - rho_G/rho) on the
+            // predicted velocity
+            const float az = ph.gz * (1.0f - ph.rho_G * inv_rho);
+#pragma unroll
+            for (int q = 0; q < Q; ++q)
+                if (CZ[q] != 0) gv[q] += 3.0f * EWF[q] * static_cast<float>(CZ[q]) * az;
+        }
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             fout[q * g.n + c] = fv[q];
@@ -300,7 +311,9 @@ __device__ __forceinline__ void decode_h(const float* __restrict__ hin, const fl
         const int sx = CX[q] == 1 ? xm : (CX[q] == -1 ? xp : x);
         const int sy = CY[q] == 1 ? ym : (CY[q] == -1 ? yp : y);
         const int sz = CZ[q] == 1 ? zm : (CZ[q] == -1 ? zp : z);
-        H[q] = EW[q] * static_cast<double>(pbase[lin(sx, sy, sz, g)]) + static_cast<double>(hin[q * g.n + c]);
+        // a value bounced at a z wall was encoded by this cell, with its base
+        const int64_t src = z_bounced_in(q, z, g) ? c : lin(sx, sy, sz, g);
+        H[q] = EW[q] * static_cast<double>(pbase[src]) + static_cast<double>(hin[q * g.n + c]);
     }
     slip(H, x, g.lx);
 }
@@ -369,7 +382,10 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_poisson_push(
         const int dx = CX[q] == 1 ? xp : (CX[q] == -1 ? xm : x);
         const int dy = CY[q] == 1 ? yp : (CY[q] == -1 ? ym : y);
         const int dz = CZ[q] == 1 ? zp : (CZ[q] == -1 ? zm : z);
-        hout[q * g.n + lin(dx, dy, dz, g)] = o;
+        if (z_bounced_out(q, z, g))
+            hout[OPP[q] * g.n + c] = o;  // bounce-back at a z wall (E_opp = E_q)
+        else
+            hout[q * g.n + lin(dx, dy, dz, g)] = o;
     }
 }
 
@@ -481,8 +497,8 @@ __global__ __launch_bounds__(BLOCK) void k_correct(Grid g, const double* __restr
     const int x_w = (x == 0) ? 1 : x - 1;
     const int y_n = (y == g.ly - 1) ? 0 : y + 1;
     const int y_s = (y == 0) ? g.ly - 1 : y - 1;
-    const int z_n = (z == g.lz - 1) ? 0 : z + 1;
-    const int z_s = (z == 0) ? g.lz - 1 : z - 1;
+    const int z_n = (z == g.lz - 1) ? (g.zwall ? g.lz - 1 : 0) : z + 1;
+    const int z_s = (z == 0) ? (g.zwall ? 0 : g.lz - 1) : z - 1;
     const float inv_2rho = 0.5f / rho[c];
     u[c] -= static_cast<float>(p[lin(x_e, y, z, g)] - p[lin(x_w, y, z, g)]) * inv_2rho;
     v[c] -= static_cast<float>(p[lin(x, y_n, z, g)] - p[lin(x, y_s, z, g)]) * inv_2rho;

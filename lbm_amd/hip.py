@@ -43,6 +43,7 @@ EXPORTED = (
     "lbm_perf_reset", "lbm_mass", "lbm_init_reference", "lbm_place_droplet",
     "lbm_reset_fields", "lbm_set_iteration_schedule", "lbm_iteration_history",
     "lbm_adjoint", "lbm_set_adjoint_memory", "lbm_adjoint_stats", "lbm_set_poisson_graph",
+    "lbm_set_z_walls", "lbm_set_gravity",
 )
 
 
@@ -132,6 +133,8 @@ def load_library():
                               dp, dp, dp]
     L.lbm_set_adjoint_memory.argtypes = [vp, ctypes.c_int64]
     L.lbm_set_poisson_graph.argtypes = [vp, ctypes.c_int]
+    L.lbm_set_z_walls.argtypes = [vp, ctypes.c_int]
+    L.lbm_set_gravity.argtypes = [vp, ctypes.c_double]
     L.lbm_adjoint_stats.argtypes = [vp, ctypes.POINTER(AdjointStats)]
     _lib = L
     return L
@@ -286,6 +289,14 @@ class InamuroHIP:
     def setUsePoissonDualResidual(self, enabled: bool):
         ci, tol, lim, _ = self._cfg()
         self.setPoisson(ci, tol, lim, enabled)
+
+    def set_z_walls(self, enabled: bool = True):
+        """No-slip walls at z = 0 and lz-1 (extension; off by default)."""
+        self._check(load_library().lbm_set_z_walls(self._h, int(bool(enabled))))
+
+    def set_gravity(self, gz: float):
+        """Gravitational acceleration along z (extension; 0 by default)."""
+        self._check(load_library().lbm_set_gravity(self._h, float(gz)))
 
     def setUsePoissonGraph(self, enabled: bool):
         """Replay each check-interval segment as a captured HIP graph (default on)."""

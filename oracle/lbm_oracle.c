@@ -55,6 +55,8 @@ static const double FW[Q] = {-7.0 / 3.0,
 /* x-wall specular pairs (+x dir, -x dir), CPU/Inamuro.cpp:592-593. */
 static const int SLIP_POS[5] = {1, 7, 9, 10, 12};
 static const int SLIP_NEG[5] = {4, 8, 14, 13, 11};
+/* Opposite direction c_opp = -c (bounce-back at the z walls, extension). */
+static const int OPP[Q] = {0, 4, 5, 6, 1, 2, 3, 11, 12, 13, 14, 7, 8, 9, 10};
 
 struct orc_sim {
     orc_params P;
@@ -64,6 +66,8 @@ struct orc_sim {
     double* arr[ORC_NUM_SCALAR_ARRAYS];
     int* xn; int* yn; int* zn;          /* stencil neighbour tables per axis, [coord*3 + (c+1)] */
     int first_step;                     /* function-static is_first_step, CPU/Inamuro.cpp:150 */
+    int z_walls;                        /* extension: no-slip walls at z = 0 and lz-1 */
+    double gz;                          /* extension: gravitational acceleration along z */
     int nthreads;
     uint64_t check_interval;
     double tol;
@@ -89,6 +93,25 @@ static void set_err(orc_sim* s, const char* msg)
 const char* orc_last_error(const orc_sim* s) { return s->err; }
 
 void orc_set_threads(orc_sim* s, int nthreads) { s->nthreads = nthreads > 0 ? nthreads : 1; }
+
+/* z neighbours of the stencils: periodic (CPU/Inamuro.hpp:240-247), or, with
+ * the z walls of the extension, the cell itself beyond a wall (zero normal
+ * gradient about a wall half a cell outside the first and last layer). */
+static void build_z_neighbours(orc_sim* s)
+{
+    for (int z = 0; z < s->lz; ++z)
+        for (int c = -1; c <= 1; ++c) {
+            int zp = z + c;
+            if (s->z_walls) {
+                if (zp >= s->lz) zp = s->lz - 1;
+                if (zp < 0) zp = 0;
+            } else {
+                if (zp >= s->lz) zp = 0;
+                if (zp < 0) zp = s->lz - 1;
+            }
+            s->zn[z * 3 + c + 1] = zp;
+        }
+}
 
 orc_sim* orc_create(const orc_params* params)
 {
@@ -128,13 +151,7 @@ orc_sim* orc_create(const orc_params* params)
             if (yp < 0) yp = s->ly - 1;
             s->yn[y * 3 + c + 1] = yp;
         }
-    for (int z = 0; z < s->lz; ++z)
-        for (int c = -1; c <= 1; ++c) {
-            int zp = z + c;
-            if (zp >= s->lz) zp = 0;
-            if (zp < 0) zp = s->lz - 1;
-            s->zn[z * 3 + c + 1] = zp;
-        }
+    build_z_neighbours(s);
     /* initializeArrays, CPU/Inamuro.cpp:511-548. */
     for (size_t i = 0; i < s->n; ++i) {
         A(s, ORC_RHO)[i] = params->rho_G;
@@ -444,6 +461,13 @@ static void collision(orc_sim* s)
                       3.0 * EW[k] / rho * mu * (UC[k] * ulap + VC[k] * vlap + WC[k] * wlap);
             }
         }
+        if (s->gz != 0.0) {
+            /* extension: buoyant acceleration a = g (1 - rho_G/rho) along z on
+             * the predicted velocity, sum_k c_k 3 E_k c_kz a = a e_z */
+            const double az = s->gz * (1.0 - P->rho_G / rho);
+            for (int k = 0; k < Q; ++k)
+                if (EZ[k] != 0) s->g[(size_t)k * n + i] += 3.0 * EW[k] * WC[k] * az;
+        }
     }
     s->first_step = 0;
 }
@@ -467,7 +491,14 @@ static void stream(orc_sim* s, double* dist)
             for (int y = 0; y < ly; ++y) {
                 const int ys = (y - EY[q] + ly) % ly;
                 for (int z = 0; z < lz; ++z) {
-                    const int zs = (z - EZ[q] + lz) % lz;
+                    const int zr = z - EZ[q];
+                    if (s->z_walls && (zr < 0 || zr >= lz)) {
+                        /* extension: halfway bounce-back, the population that
+                         * left this cell towards the wall returns reversed */
+                        dst[IX(s, x, y, z)] = dist[(size_t)OPP[q] * n + IX(s, x, y, z)];
+                        continue;
+                    }
+                    const int zs = (zr + lz) % lz;
                     dst[IX(s, x, y, z)] = src[IX(s, xs, ys, zs)];
                 }
             }
@@ -735,6 +766,17 @@ static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p)
     return 0;
 }
 
+/* Extension: no-slip z walls and gravity. */
+int orc_set_z_walls(orc_sim* s, int enabled)
+{
+    if (enabled && s->lz < 2) { set_err(s, "z walls need lz >= 2"); return -1; }
+    s->z_walls = enabled ? 1 : 0;
+    build_z_neighbours(s);
+    return 0;
+}
+
+void orc_set_gravity(orc_sim* s, double gz) { s->gz = gz; }
+
 /* Extension: freeze the iteration count of the next `count` solves. */
 int orc_set_schedule(orc_sim* s, const uint64_t* iters, int count)
 {
@@ -772,8 +814,8 @@ static void correct_uvw(orc_sim* s)
                 const int x_w = (x == 0) ? 1 : x - 1;
                 const int y_n = (y == ly - 1) ? 0 : y + 1;
                 const int y_s = (y == 0) ? ly - 1 : y - 1;
-                const int z_n = (z == lz - 1) ? 0 : z + 1;
-                const int z_s = (z == 0) ? lz - 1 : z - 1;
+                const int z_n = (z == lz - 1) ? (s->z_walls ? lz - 1 : 0) : z + 1;
+                const int z_s = (z == 0) ? (s->z_walls ? 0 : lz - 1) : z - 1;
                 const size_t i = IX(s, x, y, z);
                 const double inv_2rho = 1.0 / (2.0 * rho[i]);
                 u[i] -= (p[IX(s, x_e, y, z)] - p[IX(s, x_w, y, z)]) * inv_2rho;
@@ -1000,6 +1042,10 @@ int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
     const int lx = s->lx, ly = s->ly, lz = s->lz;
     int rc = -1;
     uint64_t max_it = 0;
+    if (s->z_walls || s->gz != 0.0) {
+        set_err(s, "the adjoint covers the reference boundaries without gravity");
+        return -1;
+    }
     for (int k = 0; k < steps; ++k) max_it = iters[k] > max_it ? iters[k] : max_it;
 
     adj_ckpt* ck = (adj_ckpt*)calloc((size_t)(steps > 0 ? steps : 1), sizeof(adj_ckpt));

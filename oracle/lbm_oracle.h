@@ -141,6 +141,14 @@ int orc_objective(orc_sim* s, const double* phi0, const double* target, int step
 int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
                 const uint64_t* iters, double* J, double* grad_phi0, double* grad_kg);
 
+/* Extension (no counterpart in the reference): no-slip walls at z = 0 and
+ * z = lz-1 -- halfway bounce-back of f, g and h in the streaming step, zero
+ * normal gradient in the stencils and in correct_uvw -- and a gravitational
+ * acceleration gz along z, applied to the predicted velocity as the buoyant
+ * acceleration gz (1 - rho_G/rho).  Both off by default. */
+int orc_set_z_walls(orc_sim* s, int enabled);
+void orc_set_gravity(orc_sim* s, double gz);
+
 /* Closed-form helpers exposed for unit tests. */
 double orc_eos(const orc_params* p, double fei);                 /* CPU/Inamuro.cpp:843-847 */
 double orc_rho_of_phi(const orc_params* p, double fei);           /* CPU/Inamuro.cpp:313-327 */

@@ -131,6 +131,8 @@ def lib():
         L.orc_poisson_iterations.argtypes = [vp, ctypes.c_int]
         u64p = ctypes.POINTER(ctypes.c_uint64)
         L.orc_set_schedule.argtypes = [vp, u64p, ctypes.c_int]
+        L.orc_set_z_walls.argtypes = [vp, ctypes.c_int]
+        L.orc_set_gravity.argtypes = [vp, ctypes.c_double]
         L.orc_objective.argtypes = [vp, dp, dp, ctypes.c_int, u64p, dp]
         L.orc_adjoint.argtypes = [vp, dp, dp, ctypes.c_int, u64p, dp, dp, dp]
         _lib = L
@@ -234,6 +236,14 @@ class Oracle:
 
     def poisson_iterations(self, k: int) -> None:
         lib().orc_poisson_iterations(self.h, k)
+
+    def set_z_walls(self, enabled: bool = True) -> None:
+        """No-slip walls at z = 0 and lz-1 (extension; off by default)."""
+        self._check(lib().orc_set_z_walls(self.h, int(bool(enabled))))
+
+    def set_gravity(self, gz: float) -> None:
+        """Gravitational acceleration along z (extension; 0 by default)."""
+        lib().orc_set_gravity(self.h, float(gz))
 
     def set_schedule(self, iters) -> None:
         """Freeze the iteration counts of the next len(iters) Poisson solves."""
