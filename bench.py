@@ -56,6 +56,9 @@ WORKLOADS = {
     # two R=24 droplets at x=80 and x=176 moving at -/+0.02 towards each other,
     # radii perturbed by seeded real spherical harmonics 2<=l<=8 (RMS 0.5)
     "head_on_collision": dict(shape=(128, 128, 256), R=24, head_on=True),
+    # R=32 droplet released at z=300 in a 192x192x384 box between no-slip z
+    # walls, falling under gravity 1e-5 (lattice units)
+    "falling_droplet": dict(shape=(384, 192, 192), R=32, cz=300.0, zwalls=True, gz=-1e-5),
 }
 
 
@@ -77,11 +80,15 @@ def make_sim(hip, wl, precision):
     P = workload_params(hip, wl)
     sim = hip.InamuroHIP(P, precision={"fp64": hip.FP64, "mixed": hip.MIXED}[precision])
     sim.setUsePoissonGraph(GRAPHS)
+    if wl.get("zwalls"):
+        sim.set_z_walls(True)
+    if wl.get("gz"):
+        sim.set_gravity(wl["gz"])
     fields = initial_fields(wl, P)
     if fields is None:
         lz, ly, lx = wl["shape"]
         sim.reset_fields()
-        sim.place_droplet(lx / 2.0, ly / 2.0, lz / 2.0, profile="tanh", width=2.0)
+        sim.place_droplet(lx / 2.0, ly / 2.0, wl.get("cz", lz / 2.0), profile="tanh", width=2.0)
     else:
         sim.upload_fields(fields)
     return sim, P
@@ -229,9 +236,13 @@ def cpu_baseline(hip, wl, workload, n_it_per_step):
     lz, ly, lx = wl["shape"]
     P = po.Params(lx=lx, ly=ly, lz=lz, DD=2.0 * wl["R"], **PHYSICS)
     o = po.Oracle(P, threads=1)
+    if wl.get("zwalls"):
+        o.set_z_walls(True)
+    if wl.get("gz"):
+        o.set_gravity(wl["gz"])
     fields = initial_fields(wl, P)
     if fields is None:
-        o.place_droplet(lx / 2.0, ly / 2.0, lz / 2.0, profile="tanh", width=2.0)
+        o.place_droplet(lx / 2.0, ly / 2.0, wl.get("cz", lz / 2.0), profile="tanh", width=2.0)
     else:
         for k, v in fields.items():
             o.set(k, v)
@@ -329,6 +340,8 @@ def main(argv=None):
             "workload": (f"{args.workload}: {lx}x{ly}x{lz} D3Q15 grid, "
                          + ("two tanh droplets R=24 at x=80/176, velocities +/-0.02, radii perturbed by seeded "
                             "spherical harmonics 2<=l<=8 (RMS 0.5)" if wl.get("head_on") else
+                            f"tanh droplet R={wl['R']} released at z={wl['cz']:g} between no-slip z walls, "
+                            f"gravity {wl['gz']:g}" if wl.get("zwalls") else
                             f"tanh droplet R={wl['R']} at the centre")
                          + ", density ratio 1000, Poisson tol 1e-3 / check 100, steps after a cold start"),
             "grid": [lx, ly, lz], "radius": wl["R"], "precision": head["precision"],
