@@ -184,7 +184,7 @@ int lbm_set_poisson_graph(void* ctx, int enabled);
  * lbm_set_gravity: gravitational acceleration gz along z (negative = down),
  *   applied to the predicted velocity as the buoyant acceleration
  *   gz (1 - rho_G/rho).
- * The adjoint covers the reference configuration only (neither set). */
+ * lbm_adjoint transposes both. */
 int lbm_set_z_walls(void* ctx, int enabled);
 int lbm_set_gravity(void* ctx, double gz);
 
@@ -226,7 +226,8 @@ int lbm_perf_reset(void* ctx);
 int lbm_mass(void* ctx, double* sum_phi);
 
 /* Discrete adjoint (no counterpart in the reference; FP64 contexts only).
- * Objective J = sum (phi_T - target)^2 after `steps` steps from phi0 with
+ * Objective J = sum (phi_T - target)^2 (or lbm_set_adjoint_objective's
+ * centroid height) after `steps` steps from phi0 with
  * rho0 = rho_G + (phi0 - fei_min)(rho_L - rho_G)/(fei_max - fei_min) (the
  * density rule of place_droplet, GPU/Inamuro.cpp:521-527), u = v = w = p = 0
  * and the first-step flag set (CPU/Inamuro.cpp:150-151).  Returns J, dJ/dphi0
@@ -240,6 +241,14 @@ int lbm_mass(void* ctx, double* sum_phi);
 int lbm_adjoint(void* ctx, const double* phi0, const double* target, int32_t steps,
                 int32_t checkpoint_interval, const uint64_t* iterations, double* objective,
                 double* grad_phi0, double* grad_kg);
+
+/* Objective of lbm_adjoint (no counterpart in the reference):
+ *   LBM_OBJ_MISFIT      J = sum (phi_T - target)^2 (the default)
+ *   LBM_OBJ_CENTROID_Z  J = sum z w / sum w, w = phi_T - fei_G, z the layer
+ *                       index: the droplet's centroid height (target unused,
+ *                       may be NULL). */
+enum { LBM_OBJ_MISFIT = 0, LBM_OBJ_CENTROID_Z = 1 };
+int lbm_set_adjoint_objective(void* ctx, int32_t kind);
 
 /* Device memory for recorded Poisson iterates in the reverse sweep (0 =
  * automatic, 90% of what is free beyond 2 GiB).  A solve longer than the

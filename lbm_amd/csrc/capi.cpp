@@ -103,6 +103,7 @@ struct Ctx {
     std::vector<uint64_t> schedule;  // frozen per-step iteration counts (empty = converge)
     size_t sched_pos = 0;
     int64_t adj_store_bytes = 0;     // device bytes for recorded Poisson iterates (0 = automatic)
+    int adj_objective = lbm::ADJ_OBJ_MISFIT;
     lbm_adjoint_stats_t adj_stats{};
 
     // Captured Poisson segments (setUsePoissonGraph): one graph per entry
@@ -966,9 +967,8 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
                  double* J, double* grad_phi0, double* grad_kg)
 {
     if (c.mixed) throw ApiError{LBM_ERR_ARG, "the adjoint runs on an FP64 context"};
-    if (c.g.zwall || c.ph.gz != 0.0)
-        throw ApiError{LBM_ERR_ARG, "the adjoint covers the reference boundaries without gravity"};
-    if (!phi0 || !target || !J || T < 1 || K < 1)
+    const bool misfit = c.adj_objective == lbm::ADJ_OBJ_MISFIT;
+    if (!phi0 || (misfit && !target) || !J || T < 1 || K < 1)
         throw ApiError{LBM_ERR_ARG, "adjoint needs phi0, target, objective, steps >= 1 and checkpoint interval >= 1"};
     const uint64_t ci = static_cast<uint64_t>(c.check_interval);
     uint64_t sched_max = 0;
@@ -1003,7 +1003,7 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
         a.cg[i] = dalloc(lbm::Q * n);
     }
     a.pc = dalloc(lbm::adj::PC_SLOTS * n);
-    a.parts = dalloc(static_cast<size_t>(lbm::grid_blocks(g.n)));
+    a.parts = dalloc(2 * static_cast<size_t>(lbm::grid_blocks(g.n)));
     a.as = static_cast<lbm::AdjState*>(dalloc_bytes(sizeof(lbm::AdjState)));
     HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&a.has), sizeof(lbm::AdjState), hipHostMallocDefault));
 
@@ -1082,8 +1082,8 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     for (uint64_t v : sched) stats.forward_iterations += v;
 
     // Seed: J and dJ/dfei_T; every other adjoint starts at zero.
-    HIPCHK(hipMemcpyAsync(a.target, target, n * sizeof(double), hipMemcpyHostToDevice, s));
-    lbm::adj::seed(s, g, D<double>(c.fei), a.target, a.lfei, a.parts, a.as);
+    if (misfit) HIPCHK(hipMemcpyAsync(a.target, target, n * sizeof(double), hipMemcpyHostToDevice, s));
+    lbm::adj::seed(s, g, c.ph, c.adj_objective, D<double>(c.fei), a.target, a.lfei, a.parts, a.as);
     for (double* p : {a.lrho, a.lu, a.lv, a.lw, a.lp}) HIPCHK(hipMemsetAsync(p, 0, n * sizeof(double), s));
     HIPCHK(hipMemsetAsync(a.cf[0], 0, slot_d * sizeof(double), s));
     HIPCHK(hipMemsetAsync(a.cg[0], 0, slot_d * sizeof(double), s));
@@ -1479,6 +1479,15 @@ int lbm_adjoint(void* ctx, const double* phi0, const double* target, int32_t ste
 {
     return guarded(ctx, [&](Ctx& c) {
         run_adjoint(c, phi0, target, steps, checkpoint_interval, iterations, objective, grad_phi0, grad_kg);
+    });
+}
+
+int lbm_set_adjoint_objective(void* ctx, int32_t kind)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        if (kind != LBM_OBJ_MISFIT && kind != LBM_OBJ_CENTROID_Z)
+            throw ApiError{LBM_ERR_ARG, "unknown adjoint objective"};
+        c.adj_objective = kind == LBM_OBJ_CENTROID_Z ? lbm::ADJ_OBJ_CENTROID_Z : lbm::ADJ_OBJ_MISFIT;
     });
 }
 

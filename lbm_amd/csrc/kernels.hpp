@@ -114,14 +114,18 @@ struct AdjState {
     double lam_shift;   // pending gauge adjoint Lambda/N, applied on read
     double lam_target;  // adjoint of the gauge target mean(p_start), this step
     double lS_mean;     // mean of the source adjoint (projection S - mean)
-    double pad;
+    double weight;      // centroid objective: sum of the weights phi - fei_G
 };
+// Objective of the adjoint: sum (phi_T - target)^2, or the centroid height
+// sum z w / sum w with w = phi_T - fei_G and z the layer index.
+enum : int { ADJ_OBJ_MISFIT = 0, ADJ_OBJ_CENTROID_Z = 1 };
 enum : int { ADJ_SRC_PLAIN = 0, ADJ_SRC_LAZY = 1, ADJ_SRC_INIT = 2 };
 
 namespace adj {
 
-void seed(hipStream_t s, const Grid& g, const double* fei, const double* target, double* lfei,
-          double* parts, AdjState* as);
+// parts holds 2 * grid_blocks(g.n) doubles.
+void seed(hipStream_t s, const Grid& g, const Phys& ph, int objective, const double* fei,
+          const double* target, double* lfei, double* parts, AdjState* as);
 void correct(hipStream_t s, const Grid& g, const Phys& ph, const double* p, const double* rho,
              const double* lu, const double* lv, const double* lw, const double* lp, double* lrho,
              double* lpf, double* parts, AdjState* as);

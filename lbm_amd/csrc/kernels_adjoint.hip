@@ -35,13 +35,25 @@ namespace {
 // from the slip pairs; -1 for directions with c_x = 0.
 constexpr int PARTNER[Q] = {-1, 4, -1, -1, 1, -1, -1, 8, 7, 14, 13, 12, 11, 10, 9};
 
-// Transpose of "pull from x - c_q, then x-slip overwrite" (the view every
-// reader of f, g and h has).  view(q, d) is the adjoint of the viewed value of
-// direction q at cell d; out[r] becomes the adjoint of the unstreamed value r
-// at this cell, which streams to d = x + c_r:
+// Transpose of "pull from x - c_q (across a z wall: the cell's own opposite
+// population), then x-slip overwrite" (the view every reader of f, g and h
+// has).  view(q, d) is the adjoint of the viewed value of direction q at cell
+// d; out[r] becomes the adjoint of the unstreamed value r at this cell, which
+// lands at d = x + c_r as direction r, or, when that crosses a z wall, at this
+// cell as direction opp(r).  Where it lands as direction rr at d:
 //   d on the x = 0 wall:      +x values there are overwritten  -> 0
-//                             -x value r also feeds its partner -> + view(partner)
+//                             -x value rr also feeds its partner -> + view(partner)
 //   d on the x = lx-1 wall:   the same with the signs exchanged.
+template <typename View>
+__device__ __forceinline__ double slip_T(int rr, int dx, int64_t d, const Grid& g, View view)
+{
+    if (CX[rr] == 0) return view(rr, d);
+    const bool low = dx == 0, high = dx == g.lx - 1;
+    if ((low && CX[rr] == 1) || (high && CX[rr] == -1)) return 0.0;
+    if (low || high) return view(rr, d) + view(PARTNER[rr], d);
+    return view(rr, d);
+}
+
 template <typename View>
 __device__ __forceinline__ void gather_stream_slip_T(int x, int y, int z, const Grid& g, View view,
                                                      double (&out)[Q])
@@ -51,23 +63,27 @@ __device__ __forceinline__ void gather_stream_slip_T(int x, int y, int z, const 
     const int zm = wrap(z, -1, g.lz), zp = wrap(z, 1, g.lz);
 #pragma unroll
     for (int r = 0; r < Q; ++r) {
-        const int dx = CX[r] == 1 ? xp : (CX[r] == -1 ? xm : x);
-        const int dy = CY[r] == 1 ? yp : (CY[r] == -1 ? ym : y);
-        const int dz = CZ[r] == 1 ? zp : (CZ[r] == -1 ? zm : z);
-        const int64_t d = lin(dx, dy, dz, g);
-        if (CX[r] == 0) {
-            out[r] = view(r, d);
+        if (z_bounced_out(r, z, g)) {
+            out[r] = slip_T(OPP[r], x, lin(x, y, z, g), g, view);
         } else {
-            const bool low = dx == 0, high = dx == g.lx - 1;
-            if ((low && CX[r] == 1) || (high && CX[r] == -1)) {
-                out[r] = 0.0;
-            } else if (low || high) {
-                out[r] = view(r, d) + view(PARTNER[r], d);
-            } else {
-                out[r] = view(r, d);
-            }
+            const int dx = CX[r] == 1 ? xp : (CX[r] == -1 ? xm : x);
+            const int dy = CY[r] == 1 ? yp : (CY[r] == -1 ? ym : y);
+            const int dz = CZ[r] == 1 ? zp : (CZ[r] == -1 ? zm : z);
+            out[r] = slip_T(r, dx, lin(dx, dy, dz, g), g, view);
         }
     }
+}
+
+// How many viewed values read the unstreamed value r at cell x (0, 1 or 2):
+// the x-slip count of where it lands (see slip_T).
+__device__ __forceinline__ double view_count(int r, int x, int z, const Grid& g)
+{
+    const bool b = z_bounced_out(r, z, g);
+    const int rr = b ? OPP[r] : r;
+    const int dx = b ? x : wrap(x, CX[r], g.lx);
+    if (CX[rr] == 1) return dx == 0 ? 0.0 : (dx == g.lx - 1 ? 2.0 : 1.0);
+    if (CX[rr] == -1) return dx == g.lx - 1 ? 0.0 : (dx == 0 ? 2.0 : 1.0);
+    return 1.0;
 }
 
 // The 15 streamed values of cell c seen through the x-slip rule (the forward
@@ -127,6 +143,53 @@ __global__ __launch_bounds__(BLOCK) void k_adj_set_objective(const double* __res
     }
 }
 
+// Centroid height J = sum z w / sum w, w = fei - fei_G: block partials of
+// sum w (parts[0..nb)) and sum z w (parts[nb..2nb)).
+__global__ __launch_bounds__(BLOCK) void k_adj_centroid_parts(Grid g, double fei_G,
+                                                              const double* __restrict__ fei,
+                                                              double* __restrict__ parts)
+{
+    __shared__ double lds[4];
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    double w = 0.0, zw = 0.0;
+    if (c < g.n) {
+        int x, y, z;
+        cell_xyz(c, g, x, y, z);
+        w = fei[c] - fei_G;
+        zw = static_cast<double>(z) * w;
+    }
+    const double sw = block_sum(w, lds);
+    const double szw = block_sum(zw, lds);
+    if (threadIdx.x == 0) {
+        parts[blockIdx.x] = sw;
+        parts[gridDim.x + blockIdx.x] = szw;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_adj_centroid_final(const double* __restrict__ parts,
+                                                              int nparts, AdjState* as)
+{
+    __shared__ double lds[4];
+    const double W = reduce_partials(parts, nparts, lds);
+    const double Z = reduce_partials(parts + nparts, nparts, lds);
+    if (threadIdx.x == 0) {
+        as->J = Z / W;
+        as->weight = W;
+        as->kg = 0.0;
+    }
+}
+
+// dJ/dfei = (z - J) / W.
+__global__ __launch_bounds__(BLOCK) void k_adj_centroid_seed(Grid g, const AdjState* __restrict__ as,
+                                                             double* __restrict__ lfei)
+{
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    if (c >= g.n) return;
+    int x, y, z;
+    cell_xyz(c, g, x, y, z);
+    lfei[c] = (static_cast<double>(z) - as->J) / as->weight;
+}
+
 // ---- correct_uvw --------------------------------------------------------------
 
 // u -= (p_e - p_w) / (2 rho') and likewise v, w (CPU/Inamuro.cpp:794-823).
@@ -162,7 +225,19 @@ __global__ __launch_bounds__(BLOCK) void k_adj_correct(
             acc -= lv[is] * (1.0 / (2.0 * rho[is]));
             acc += lv[in] * (1.0 / (2.0 * rho[in]));
         }
-        if (g.lz > 2) {
+        if (g.zwall) {
+            // a wall layer's outer neighbour is the layer itself
+            if (z >= 1) {
+                const int64_t ib = lin(x, y, z - 1, g);
+                acc -= lw[ib] * (1.0 / (2.0 * rho[ib]));
+            }
+            if (z == g.lz - 1) acc -= lw[c] * (1.0 / (2.0 * rho[c]));
+            if (z <= g.lz - 2) {
+                const int64_t it = lin(x, y, z + 1, g);
+                acc += lw[it] * (1.0 / (2.0 * rho[it]));
+            }
+            if (z == 0) acc += lw[c] * (1.0 / (2.0 * rho[c]));
+        } else if (g.lz > 2) {
             const int64_t ib = lin(x, y, wrap(z, -1, g.lz), g), it = lin(x, y, wrap(z, 1, g.lz), g);
             acc -= lw[ib] * (1.0 / (2.0 * rho[ib]));
             acc += lw[it] * (1.0 / (2.0 * rho[it]));
@@ -174,8 +249,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_correct(
         const int x_w = (x == 0) ? 1 : x - 1;
         const int y_n = (y == g.ly - 1) ? 0 : y + 1;
         const int y_s = (y == 0) ? g.ly - 1 : y - 1;
-        const int z_n = (z == g.lz - 1) ? 0 : z + 1;
-        const int z_s = (z == 0) ? g.lz - 1 : z - 1;
+        const int z_n = z_stencil(z, 1, g);
+        const int z_s = z_stencil(z, -1, g);
         const double r = rho[c];
         lrho[c] += (lu[c] * (p[lin(x_e, y, z, g)] - p[lin(x_w, y, z, g)]) +
                     lv[c] * (p[lin(x, y_n, z, g)] - p[lin(x, y_s, z, g)]) +
@@ -248,14 +323,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
             if (SRC == SRC_LAZY) {
                 const double ls = as->lam_shift;
 #pragma unroll
-                for (int q = 0; q < Q; ++q) {
-                    // multiplicity of the viewed values behind lh*_q(x) at
-                    // y = x + c_q
-                    double m = 1.0;
-                    if (CX[q] == 1) m = x == g.lx - 1 ? 0.0 : (x == g.lx - 2 ? 2.0 : 1.0);
-                    if (CX[q] == -1) m = x == 0 ? 0.0 : (x == 1 ? 2.0 : 1.0);
-                    lh[q] -= m * ls;
-                }
+                for (int q = 0; q < Q; ++q) lh[q] -= view_count(q, x, z, g) * ls;
             }
         }
         double hv[Q];
@@ -319,10 +387,16 @@ __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
             const int zm = wrap(z, -1, g.lz), zp = wrap(z, 1, g.lz);
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                const int sx = CX[q] == 1 ? xm : (CX[q] == -1 ? xp : x);
-                const int sy = CY[q] == 1 ? ym : (CY[q] == -1 ? yp : y);
-                const int sz = CZ[q] == 1 ? zm : (CZ[q] == -1 ? zp : z);
-                Bout[q * g.n + lin(sx, sy, sz, g)] = lh[q];
+                // the streamed value q at x came from x - c_q, or, across a
+                // z wall, from this cell's opposite population
+                if (z_bounced_in(q, z, g)) {
+                    Bout[OPP[q] * g.n + c] = lh[q];
+                } else {
+                    const int sx = CX[q] == 1 ? xm : (CX[q] == -1 ? xp : x);
+                    const int sy = CY[q] == 1 ? ym : (CY[q] == -1 ? yp : y);
+                    const int sz = CZ[q] == 1 ? zm : (CZ[q] == -1 ? zp : z);
+                    Bout[q * g.n + lin(sx, sy, sz, g)] = lh[q];
+                }
             }
         }
     }
@@ -419,7 +493,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_collide(
         int64_t idx[Q];
 #pragma unroll
         for (int k = 0; k < Q; ++k)
-            idx[k] = lin(mirror_x(x, CX[k], g.lx), wrap(y, CY[k], g.ly), wrap(z, CZ[k], g.lz), g);
+            idx[k] = lin(mirror_x(x, CX[k], g.lx), wrap(y, CY[k], g.ly), z_stencil(z, CZ[k], g), g);
         double Lu[3], gp[3], gr[3], Lphi;
         {
             double au = 0.0, av = 0.0, aw = 0.0, ax = 0.0, ay = 0.0, az = 0.0;
@@ -460,6 +534,8 @@ __global__ __launch_bounds__(BLOCK) void k_adj_collide(
         for (int q = 0; q < Q; ++q) {
             const double cq[3] = {UC[q], VC[q], WC[q]};
             double fq = af[q], gq = ag[q];
+            // gravity: g_q += 3 E_q c_qz gz (1 - rho_G/rho)
+            if (CZ[q] != 0 && ph.gz != 0.0) a_rho += gq * 3.0 * EW[q] * WC[q] * ph.gz * ph.rho_G / (rc * rc);
             if (FIRST) {
                 cf_o[q * g.n + c] = 0.0;
                 cg_o[q * g.n + c] = 0.0;
@@ -560,7 +636,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_kg(const double* __restrict__ par
 // Transpose of firstord / secondord (CPU/Inamuro.hpp:213-314): the cells x'
 // whose stencil direction k reads cell x are x - c_k (y, z periodic) when that
 // lies inside [0, lx-1], plus the mirrored reader (x' = lx-1 reads lx-2 for
-// c_x = +1, x' = 0 reads 1 for c_x = -1).  Each output is the centre adjoint
+// c_x = +1, x' = 0 reads 1 for c_x = -1); with the z walls, the same in z
+// about the wall layers (x' with z' = lz-1 reads itself for c_z = +1, z' = 0
+// for c_z = -1).  Each output is the centre adjoint
 // plus sum over readers of (c_k . grad adjoint)/10 + lap adjoint/5, minus
 // 14/5 of its own lap adjoint.
 __global__ __launch_bounds__(BLOCK) void k_adj_stencil(Grid g, const double* __restrict__ pc,
@@ -588,13 +666,24 @@ __global__ __launch_bounds__(BLOCK) void k_adj_stencil(Grid g, const double* __r
         o[4] += (ck[0] * pc[PC_WX * n + s] + ck[1] * pc[PC_WY * n + s] + ck[2] * pc[PC_WZ * n + s]) / 10.0 +
                 pc[PC_WLAP * n + s] / 5.0;
     };
-#pragma unroll
-    for (int k = 1; k < Q; ++k) {
-        const int sy = wrap(y, -CY[k], g.ly), sz = wrap(z, -CZ[k], g.lz);
+    auto add_x = [&](int k, int sy, int sz) {
         const int sx = x - CX[k];
         if (sx >= 0 && sx <= g.lx - 1) add(k, lin(sx, sy, sz, g));
         if (CX[k] == 1 && x == g.lx - 2) add(k, lin(g.lx - 1, sy, sz, g));
         if (CX[k] == -1 && x == 1) add(k, lin(0, sy, sz, g));
+    };
+#pragma unroll
+    for (int k = 1; k < Q; ++k) {
+        const int sy = wrap(y, -CY[k], g.ly);
+        if (!g.zwall) {
+            add_x(k, sy, wrap(z, -CZ[k], g.lz));
+        } else {
+            // z readers with the walls: z - c_z inside the box, and a wall
+            // layer, whose stencil point beyond the wall is itself
+            const int sz = z - CZ[k];
+            if (sz >= 0 && sz <= g.lz - 1) add_x(k, sy, sz);
+            if ((CZ[k] == 1 && z == g.lz - 1) || (CZ[k] == -1 && z == 0)) add_x(k, sy, z);
+        }
     }
     lfei[c] = pc[PC_FEI * n + c] + o[0] - 14.0 * pc[PC_FLAP * n + c] / 5.0;
     lrho[c] = pc[PC_RHO * n + c] + o[1];
@@ -605,10 +694,16 @@ __global__ __launch_bounds__(BLOCK) void k_adj_stencil(Grid g, const double* __r
 
 } // namespace
 
-void seed(hipStream_t s, const Grid& g, const double* fei, const double* target, double* lfei,
-          double* parts, AdjState* as)
+void seed(hipStream_t s, const Grid& g, const Phys& ph, int objective, const double* fei,
+          const double* target, double* lfei, double* parts, AdjState* as)
 {
     const int nb = grid_blocks(g.n);
+    if (objective == ADJ_OBJ_CENTROID_Z) {
+        k_adj_centroid_parts<<<nb, BLOCK, 0, s>>>(g, ph.fei_G, fei, parts);
+        k_adj_centroid_final<<<1, BLOCK, 0, s>>>(parts, nb, as);
+        k_adj_centroid_seed<<<nb, BLOCK, 0, s>>>(g, as, lfei);
+        return;
+    }
     k_adj_seed<<<nb, BLOCK, 0, s>>>(g, fei, target, lfei, parts);
     k_adj_set_objective<<<1, BLOCK, 0, s>>>(parts, nb, as);
 }

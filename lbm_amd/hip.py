@@ -43,8 +43,11 @@ EXPORTED = (
     "lbm_perf_reset", "lbm_mass", "lbm_init_reference", "lbm_place_droplet",
     "lbm_reset_fields", "lbm_set_iteration_schedule", "lbm_iteration_history",
     "lbm_adjoint", "lbm_set_adjoint_memory", "lbm_adjoint_stats", "lbm_set_poisson_graph",
-    "lbm_set_z_walls", "lbm_set_gravity",
+    "lbm_set_z_walls", "lbm_set_gravity", "lbm_set_adjoint_objective",
 )
+
+# Objectives of lbm_set_adjoint_objective.
+OBJECTIVES = {"misfit": 0, "centroid_z": 1}
 
 
 class LbmParams(ctypes.Structure):
@@ -135,6 +138,7 @@ def load_library():
     L.lbm_set_poisson_graph.argtypes = [vp, ctypes.c_int]
     L.lbm_set_z_walls.argtypes = [vp, ctypes.c_int]
     L.lbm_set_gravity.argtypes = [vp, ctypes.c_double]
+    L.lbm_set_adjoint_objective.argtypes = [vp, ctypes.c_int32]
     L.lbm_adjoint_stats.argtypes = [vp, ctypes.POINTER(AdjointStats)]
     _lib = L
     return L
@@ -199,6 +203,7 @@ class InamuroHIP:
             _raise(rc, L.lbm_last_error(None).decode())
         self._h = h
         self.shape = (self._cp.lz, self._cp.ly, self._cp.lx)
+        self._objective = "misfit"
 
     # -- lifetime -----------------------------------------------------------
     def close(self):
@@ -360,12 +365,20 @@ class InamuroHIP:
         c = self._cp
         return c.rho_G + (np.asarray(phi0, dtype=np.float64) - c.fei_min) * (c.rho_L - c.rho_G) / (c.fei_max - c.fei_min)
 
+    def set_adjoint_objective(self, kind: str):
+        """'misfit' (J = sum (phi_T - target)^2, the default) or 'centroid_z'
+        (J = sum z w / sum w, w = phi_T - fei_G: the droplet's centroid
+        height; no target)."""
+        self._check(load_library().lbm_set_adjoint_objective(self._h, OBJECTIVES[kind]))
+        self._objective = kind
+
     def adjoint(self, phi0, target, steps: int, checkpoint_interval: int = 100, iterations=None):
-        """(J, dJ/dphi0, dJ/dk_g) of J = sum (phi_T - target)^2 after `steps`
-        steps from phi0; `iterations` freezes the Poisson counts, otherwise
-        the forward pass's own counts are frozen for the reverse sweep."""
+        """(J, dJ/dphi0, dJ/dk_g) of the objective (set_adjoint_objective)
+        after `steps` steps from phi0; `iterations` freezes the Poisson counts,
+        otherwise the forward pass's own counts are frozen for the reverse
+        sweep.  target may be None for the centroid objective."""
         phi0 = self._field(phi0)
-        target = self._field(target)
+        target = None if target is None else self._field(target)
         grad = np.empty(self.shape)
         J, gk = ctypes.c_double(), ctypes.c_double()
         it_ptr = None
@@ -374,7 +387,8 @@ class InamuroHIP:
             if len(its) != steps:
                 raise ValueError("one iteration count per step")
             it_ptr = its.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64))
-        self._check(load_library().lbm_adjoint(self._h, _ptr(phi0), _ptr(target), int(steps),
+        self._check(load_library().lbm_adjoint(self._h, _ptr(phi0), None if target is None else _ptr(target),
+                                               int(steps),
                                                int(checkpoint_interval), it_ptr, ctypes.byref(J),
                                                _ptr(grad), ctypes.byref(gk)))
         return J.value, grad, gk.value
@@ -386,6 +400,9 @@ class InamuroHIP:
         self.set_iteration_schedule(iterations)
         self.performTimeStepGPU(len(iterations))
         fei = self.downloadFieldsToCPU()["fei"]
+        if self._objective == "centroid_z":
+            w = fei - self._cp.fei_G
+            return float((np.arange(self.shape[0])[:, None, None] * w).sum() / w.sum())
         return float(((fei - self._field(target)) ** 2).sum())
 
     def set_adjoint_memory(self, iterate_bytes: int):

@@ -68,6 +68,7 @@ struct orc_sim {
     int first_step;                     /* function-static is_first_step, CPU/Inamuro.cpp:150 */
     int z_walls;                        /* extension: no-slip walls at z = 0 and lz-1 */
     double gz;                          /* extension: gravitational acceleration along z */
+    int objective;                      /* extension: ORC_OBJ_* of orc_objective / orc_adjoint */
     int nthreads;
     uint64_t check_interval;
     double tol;
@@ -885,7 +886,8 @@ void orc_poisson_diag_get(const orc_sim* s, orc_poisson_diag* out) { *out = s->d
  * step at a time from it, recording that step's Poisson iterates, so memory
  * holds one step's iterates (small grids: this is a checker).
  *
- * Objective J = sum (phi_T - target)^2 after `steps` steps from phi0, with
+ * Objective J = sum (phi_T - target)^2, or the centroid height of
+ * orc_set_objective, after `steps` steps from phi0, with
  * rho0 = rho_G + (phi0 - fei_min)(rho_L - rho_G)/(fei_max - fei_min) (the
  * density rule of place_droplet, GPU/Inamuro.cpp:521-527), zero u, v, w, p,
  * h = 0 and the first-step flag set.
@@ -935,12 +937,38 @@ static void adj_reset(orc_sim* s, const double* phi0_canon)
     s->pressure_solve_call_count = 0;
 }
 
-static double adj_misfit(const orc_sim* s, const double* target)
+int orc_set_objective(orc_sim* s, int kind)
 {
+    if (kind != ORC_OBJ_MISFIT && kind != ORC_OBJ_CENTROID_Z) {
+        set_err(s, "unknown objective");
+        return -1;
+    }
+    s->objective = kind;
+    return 0;
+}
+
+/* The objective at the current fields; with seed != NULL also dJ/dfei (internal
+ * layout).  The centroid's z of internal index i is i mod lz. */
+static double adj_value(const orc_sim* s, const double* target, double* seed)
+{
+    const double* fei = A(s, ORC_FEI);
+    if (s->objective == ORC_OBJ_CENTROID_Z) {
+        double W = 0.0, Z = 0.0;
+        for (size_t i = 0; i < s->n; ++i) {
+            const double w = fei[i] - s->P.fei_G;
+            W += w;
+            Z += (double)(i % (size_t)s->lz) * w;
+        }
+        const double zc = Z / W;
+        if (seed)
+            for (size_t i = 0; i < s->n; ++i) seed[i] = ((double)(i % (size_t)s->lz) - zc) / W;
+        return zc;
+    }
     double J = 0.0;
     for (size_t i = 0; i < s->n; ++i) {
-        const double d = A(s, ORC_FEI)[i] - target[i];
+        const double d = fei[i] - target[i];
         J += d * d;
+        if (seed) seed[i] = 2.0 * d;
     }
     return J;
 }
@@ -948,20 +976,21 @@ static double adj_misfit(const orc_sim* s, const double* target)
 int orc_objective(orc_sim* s, const double* phi0, const double* target, int steps,
                   const uint64_t* iters, double* J)
 {
-    double* t = (double*)malloc(sizeof(double) * s->n);
+    double* t = (double*)calloc(s->n, sizeof(double));
     if (!t) { set_err(s, "out of memory"); return -1; }
-    load_canonical(s, ORC_P_PREV, target, t);
+    if (s->objective == ORC_OBJ_MISFIT) load_canonical(s, ORC_P_PREV, target, t);
     adj_reset(s, phi0);
     int rc = orc_set_schedule(s, iters, steps);
     for (int k = 0; rc == 0 && k < steps; ++k) rc = orc_step(s);
-    if (rc == 0) *J = adj_misfit(s, t);
+    if (rc == 0) *J = adj_value(s, t, NULL);
     orc_set_schedule(s, NULL, 0);
     free(t);
     return rc;
 }
 
-/* Transpose of stream + slipBounceBack (CPU/Inamuro.cpp:252-292, 589-623) on
- * a [q][n] adjoint array, in place. */
+/* Transpose of stream + slipBounceBack (CPU/Inamuro.cpp:252-292, 589-623),
+ * with the z bounce-back of the extension, on a [q][n] adjoint array, in
+ * place. */
 static void adj_stream_slip(orc_sim* s, double* lam)
 {
     const size_t n = s->n;
@@ -981,7 +1010,9 @@ static void adj_stream_slip(orc_sim* s, double* lam)
                 lam[(size_t)SLIP_NEG[r] * n + il] += lam[(size_t)SLIP_POS[r] * n + il];
                 lam[(size_t)SLIP_POS[r] * n + il] = 0.0;
             }
-    /* stream wrote out_q(x) = in_q(x - c_q): in_q(y) = out_q(y + c_q) */
+    /* stream wrote out_q(x) = in_q(x - c_q): in_q(y) = out_q(y + c_q); with
+     * the z walls, in_q(y) whose y + c_q lies beyond a wall came back as
+     * out_{opp q}(y) */
     for (int q = 0; q < Q; ++q) {
         const double* src = lam + (size_t)q * n;
         double* dst = s->tmp + (size_t)q * n;
@@ -989,7 +1020,13 @@ static void adj_stream_slip(orc_sim* s, double* lam)
             const int xs = (x + EX[q] + lx) % lx;
             for (int y = 0; y < ly; ++y) {
                 const int ys = (y + EY[q] + ly) % ly;
-                for (int z = 0; z < lz; ++z) dst[IX(s, x, y, z)] = src[IX(s, xs, ys, (z + EZ[q] + lz) % lz)];
+                for (int z = 0; z < lz; ++z) {
+                    const int zt = z + EZ[q];
+                    if (s->z_walls && (zt < 0 || zt >= lz))
+                        dst[IX(s, x, y, z)] = lam[(size_t)OPP[q] * n + IX(s, x, y, z)];
+                    else
+                        dst[IX(s, x, y, z)] = src[IX(s, xs, ys, (zt + lz) % lz)];
+                }
             }
         }
     }
@@ -1042,14 +1079,10 @@ int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
     const int lx = s->lx, ly = s->ly, lz = s->lz;
     int rc = -1;
     uint64_t max_it = 0;
-    if (s->z_walls || s->gz != 0.0) {
-        set_err(s, "the adjoint covers the reference boundaries without gravity");
-        return -1;
-    }
     for (int k = 0; k < steps; ++k) max_it = iters[k] > max_it ? iters[k] : max_it;
 
     adj_ckpt* ck = (adj_ckpt*)calloc((size_t)(steps > 0 ? steps : 1), sizeof(adj_ckpt));
-    double* tgt = (double*)malloc(sizeof(double) * n);
+    double* tgt = (double*)calloc(n, sizeof(double));
     double* rec_h = (double*)malloc(sizeof(double) * qn * (size_t)(max_it ? max_it : 1));
     double* rec_p = (double*)malloc(sizeof(double) * n * (size_t)(max_it ? max_it : 1));
     double* fei_new = (double*)malloc(sizeof(double) * n);
@@ -1082,7 +1115,7 @@ int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
     if (orc_set_schedule(s, iters, steps) != 0) goto done;
 
     /* ---- forward, keeping every step's start state ---- */
-    load_canonical(s, ORC_P_PREV, target, tgt);
+    if (s->objective == ORC_OBJ_MISFIT) load_canonical(s, ORC_P_PREV, target, tgt);
     adj_reset(s, phi0);
     for (int k = 0; k < steps; ++k) {
         for (int j = 0; j < 6; ++j) memcpy(ck[k].m[j], A(s, ORC_FEI + j), sizeof(double) * n);
@@ -1092,8 +1125,7 @@ int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
         ck[k].first = s->first_step;
         if (orc_step(s) != 0) goto done;
     }
-    *J = adj_misfit(s, tgt);
-    for (size_t i = 0; i < n; ++i) la[0][i] = 2.0 * (A(s, ORC_FEI)[i] - tgt[i]);
+    *J = adj_value(s, tgt, la[0]);
 
     /* ---- reverse ---- */
     for (int k = steps - 1; k >= 0; --k) {
@@ -1128,8 +1160,8 @@ int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
                     const size_t iw = IX(s, (x == 0) ? 1 : x - 1, y, z);
                     const size_t in = IX(s, x, (y == ly - 1) ? 0 : y + 1, z);
                     const size_t is = IX(s, x, (y == 0) ? ly - 1 : y - 1, z);
-                    const size_t it = IX(s, x, y, (z == lz - 1) ? 0 : z + 1);
-                    const size_t ib = IX(s, x, y, (z == 0) ? lz - 1 : z - 1);
+                    const size_t it = IX(s, x, y, (z == lz - 1) ? (s->z_walls ? lz - 1 : 0) : z + 1);
+                    const size_t ib = IX(s, x, y, (z == 0) ? (s->z_walls ? 0 : lz - 1) : z - 1);
                     const double inv = 1.0 / (2.0 * rho_new[i]);
                     lp[ie] -= la[2][i] * inv; lp[iw] += la[2][i] * inv;
                     lp[in] -= la[3][i] * inv; lp[is] += la[3][i] * inv;
@@ -1231,6 +1263,8 @@ int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
                 const double cq[3] = {UC[q], VC[q], WC[q]};
                 const size_t iq = (size_t)q * n + i;
                 double af = lf[iq], ag = lg[iq];
+                /* gravity: g_q += 3 E_q c_qz gz (1 - rho_G/rho) */
+                if (EZ[q] != 0) a_rho += ag * 3.0 * EW[q] * WC[q] * s->gz * P->rho_G / (rho * rho);
                 if (!c->first) {
                     cf[iq] = af * (1.0 - 1.0 / P->tauf);
                     cg[iq] = ag * (1.0 - 1.0 / P->taug);

@@ -141,11 +141,20 @@ int orc_objective(orc_sim* s, const double* phi0, const double* target, int step
 int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
                 const uint64_t* iters, double* J, double* grad_phi0, double* grad_kg);
 
+/* Objective of orc_objective / orc_adjoint (extension):
+ *   ORC_OBJ_MISFIT      J = sum (phi_T - target)^2 (the default)
+ *   ORC_OBJ_CENTROID_Z  J = sum z w / sum w with w = phi_T - fei_G and z the
+ *                       cell's layer index: the height of the droplet's
+ *                       centroid; target is not read (may be NULL). */
+enum { ORC_OBJ_MISFIT = 0, ORC_OBJ_CENTROID_Z = 1 };
+int orc_set_objective(orc_sim* s, int kind);
+
 /* Extension (no counterpart in the reference): no-slip walls at z = 0 and
  * z = lz-1 -- halfway bounce-back of f, g and h in the streaming step, zero
  * normal gradient in the stencils and in correct_uvw -- and a gravitational
  * acceleration gz along z, applied to the predicted velocity as the buoyant
- * acceleration gz (1 - rho_G/rho).  Both off by default. */
+ * acceleration gz (1 - rho_G/rho).  Both off by default; orc_adjoint covers
+ * both. */
 int orc_set_z_walls(orc_sim* s, int enabled);
 void orc_set_gravity(orc_sim* s, double gz);
 

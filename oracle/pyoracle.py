@@ -135,6 +135,7 @@ def lib():
         L.orc_set_gravity.argtypes = [vp, ctypes.c_double]
         L.orc_objective.argtypes = [vp, dp, dp, ctypes.c_int, u64p, dp]
         L.orc_adjoint.argtypes = [vp, dp, dp, ctypes.c_int, u64p, dp, dp, dp]
+        L.orc_set_objective.argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -250,8 +251,18 @@ class Oracle:
         a = np.ascontiguousarray(np.asarray(iters, dtype=np.uint64))
         self._check(lib().orc_set_schedule(self.h, a.ctypes.data_as(ctypes.POINTER(ctypes.c_uint64)), len(a)))
 
+    OBJECTIVES = {"misfit": 0, "centroid_z": 1}
+
+    def set_objective(self, kind: str) -> None:
+        """'misfit' (sum (phi_T - target)^2, the default) or 'centroid_z'
+        (sum z w / sum w, w = phi_T - fei_G: the droplet's centroid height;
+        the target is then ignored)."""
+        self._check(lib().orc_set_objective(self.h, self.OBJECTIVES[kind]))
+
     def _adj_args(self, phi0, target, iters):
         phi0 = np.ascontiguousarray(np.asarray(phi0, dtype=np.float64).reshape(self.shape))
+        if target is None:
+            target = np.zeros(self.shape)
         target = np.ascontiguousarray(np.asarray(target, dtype=np.float64).reshape(self.shape))
         it = np.ascontiguousarray(np.asarray(iters, dtype=np.uint64))
         dptr = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))

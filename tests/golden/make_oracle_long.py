@@ -1,9 +1,10 @@
 """Writes the long config-0 fixture from the oracle alone:
 
-    python tests/golden/make_oracle_long.py [steps]
+    python tests/golden/make_oracle_long.py [steps] [threads]
 
 The 64^3 config-0 droplet (tanh R=16, density ratio 1000) for `steps` steps
-(default 20).  The compiled reference takes over an hour for the first three
+(default 20), the oracle's per-cell loops on `threads` workers (default 1;
+its global sums stay sequential, so the output does not depend on it).  The compiled reference takes over an hour for the first three
 steps of this case, so only those three are reference-generated
 (config0_64.json, where the oracle is asserted bit-identical to it); this
 fixture continues the same trajectory with the oracle, whose first three
@@ -32,9 +33,10 @@ def sha(a):
 
 def main(argv):
     steps = int(argv[1]) if len(argv) > 1 else 20
+    threads = int(argv[2]) if len(argv) > 2 else 1
     base = json.load(open(os.path.join(OUT, "config0_64.json")))
     P = po.Params(**base["params"])
-    o = po.Oracle(P, threads=1)
+    o = po.Oracle(P, threads=threads)
     cx, cy, cz = base["init"]["center"]
     o.place_droplet(cx, cy, cz, profile=base["init"]["kind"], width=base["init"]["width"])
     init_sha = {k: sha(v) for k, v in o.fields().items()}

@@ -6,6 +6,8 @@ reverses a population at the wall; the stencils see a zero normal gradient
 about a wall half a cell out; the gravity term adds exactly the buoyant
 acceleration to the predicted velocity; with both switched off the step is
 the reference's, bit for bit (pinned by the golden fixtures)."""
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -119,10 +121,73 @@ def test_mass_is_conserved_with_walls_and_gravity():
     assert abs(o.get("fei").sum() - m0) < 1e-7 * m0
 
 
-def test_adjoint_rejects_the_extension():
-    P = params()
+def _falling_case(objective, steps=3, seed=0):
+    """A droplet low between the walls (its tanh tail reaches z = 0), with
+    gravity, x slip walls and a small density ratio so a few steps move it."""
+    P = params(lx=10, ly=8, lz=12, rho_L=10.0)
     o = po.Oracle(P)
     o.set_z_walls(True)
-    phi = o.get("fei")
-    with pytest.raises(po.OracleError, match="reference boundaries"):
-        o.adjoint(phi, phi, [100])
+    o.set_gravity(-2e-3)
+    o.set_poisson(check_interval=10)
+    o.set_objective(objective)
+    o.place_droplet(4.0, 4.0, 3.5, profile="tanh", width=1.5)
+    phi0 = o.get("fei")
+    free = po.Oracle(P)
+    free.set_z_walls(True)
+    free.set_gravity(-2e-3)
+    free.set_poisson(check_interval=10)
+    free.set("fei", phi0)
+    free.set("rho", P.rho_G + (phi0 - P.fei_min) * (P.rho_L - P.rho_G) / (P.fei_max - P.fei_min))
+    iters = free.step(steps)
+    rng = np.random.default_rng(seed)
+    target = phi0 + 0.002 * rng.standard_normal(phi0.shape)
+    return P, o, phi0, target, iters, rng
+
+
+@pytest.mark.parametrize("objective", ["misfit", "centroid_z"])
+def test_adjoint_with_walls_and_gravity_matches_central_fd(objective):
+    """The adjoint's bounce-back, wall-stencil, wall-correct and gravity
+    transposes against central differences of the oracle's forward map, for
+    the misfit and for the droplet's centroid height."""
+    P, o, phi0, target, iters, rng = _falling_case(objective)
+    J, g, gk = o.adjoint(phi0, target, iters)
+    if objective == "centroid_z":
+        w = phi0 - P.fei_G
+        z = np.arange(P.lz)[:, None, None]
+        assert 2.0 < J < 6.0 and abs(J - (z * w).sum() / w.sum()) < 1.0
+    h = 1e-7
+    for _ in range(6):
+        d = rng.standard_normal(phi0.shape)
+        fd = (o.objective(phi0 + h * d, target, iters) - o.objective(phi0 - h * d, target, iters)) / (2 * h)
+        ad = float((g * d).sum())
+        assert abs(fd - ad) <= 1e-6 * abs(fd), (fd, ad)
+    hk = 1e-6
+    res = []
+    for kg in (P.k_g + hk, P.k_g - hk):
+        x = po.Oracle(dataclasses.replace(P, k_g=kg))
+        x.set_z_walls(True)
+        x.set_gravity(-2e-3)
+        x.set_poisson(check_interval=10)
+        x.set_objective(objective)
+        res.append(x.objective(phi0, target, iters))
+    fd = (res[0] - res[1]) / (2 * hk)
+    assert abs(fd - gk) <= 1e-5 * abs(fd), (fd, gk)
+
+
+def test_centroid_objective_of_a_resting_layered_state():
+    """The centroid seed: J of a field whose excess over fei_G is a single
+    layer is that layer's index, and its gradient is (z - z_c)/W."""
+    P = params()
+    o = po.Oracle(P)
+    o.set_objective("centroid_z")
+    o.set_z_walls(True)
+    phi0 = np.full((P.lz, P.ly, P.lx), P.fei_G)
+    phi0[5] += 0.01
+    o.set_poisson(check_interval=10)
+    # zero steps: J is the centroid of phi0 itself
+    J, g, gk = o.adjoint(phi0, None, [])
+    assert abs(J - 5.0) < 1e-12
+    W = 0.01 * P.ly * P.lx
+    z = np.arange(P.lz, dtype=np.float64)[:, None, None]
+    assert np.allclose(g, np.broadcast_to((z - 5.0) / W, g.shape), rtol=0, atol=1e-12)
+    assert gk == 0.0
