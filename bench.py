@@ -21,6 +21,7 @@ HIP events on the library's stream.  `cpu_baseline` times the CPU oracle
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import platform
@@ -160,8 +161,6 @@ def adjoint_fd(hip, steps, checkpoint, directions, h=1e-6):
     """The adjoint leg's gradient against central differences of the same
     frozen-count map at full size: `directions` seeded unit-Gaussian
     directions over phi0 (step h) and the k_g direction (step h)."""
-    import dataclasses
-
     import numpy as np
     from lbm_amd import configs
 
@@ -227,6 +226,77 @@ def run_adjoint(hip, steps, checkpoint):
     )
 
 
+def run_adjoint_falling(hip, steps, checkpoint, state, fd=False, h=1e-6):
+    """The falling droplet's adjoint: dJ/dk_g (and dJ/dphi0) of its centroid
+    height after `steps` steps from the release, on the full grid with the
+    walls and gravity, the adjoint Poisson state in `state` precision.  Cost
+    ratio as in run_adjoint; with fd, central differences of the same
+    frozen-count map in k_g."""
+    wl = WORKLOADS["falling_droplet"]
+    lz, ly, lx = wl["shape"]
+
+    def sim_for(P):
+        sim = hip.InamuroHIP(P)
+        sim.setUsePoissonGraph(GRAPHS)
+        sim.set_z_walls(True)
+        sim.set_gravity(wl["gz"])
+        sim.set_adjoint_objective("centroid_z")
+        return sim
+
+    P = workload_params(hip, wl)
+    sim = sim_for(P)
+    sim.reset_fields()
+    sim.place_droplet(lx / 2.0, ly / 2.0, wl["cz"], profile="tanh", width=2.0)
+    phi0 = sim.downloadFieldsToCPU()["fei"]
+    # the adjoint's start: rho linear in phi0, at rest
+    sim.initFromCPU(fei=phi0, rho=sim.initial_rho(phi0), u=0.0, v=0.0, w=0.0, p=0.0)
+    sim.synchronize()
+    t0 = time.perf_counter()
+    for k in range(steps):
+        sim.performTimeStepGPU(1)
+        sim.synchronize()
+        print(f"falling adjoint leg: plain forward step {k + 1}: "
+              f"{sim.getLastPoissonDiagnostics().iterations} iterations, "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    fwd_ms = (time.perf_counter() - t0) * 1e3
+    iters = sim.iteration_history()[-steps:]
+    sim.set_adjoint_state_precision(state)
+    last = [time.perf_counter()]
+
+    def progress(phase, step, it):
+        now = time.perf_counter()
+        if phase != "reverse block" or now - last[0] > 20.0:
+            last[0] = now
+            print(f"falling adjoint leg: {phase} {step} at iteration {it}, {now - t0:.1f} s",
+                  file=sys.stderr, flush=True)
+
+    sim.set_progress(progress)
+    J, g, gk = sim.adjoint(phi0, None, steps, checkpoint_interval=checkpoint, iterations=iters)
+    st = sim.adjoint_stats()
+    sim.close()
+    rec = dict(
+        workload=(f"falling_droplet: {lx}x{ly}x{lz}, tanh droplet R={wl['R']} released at z={wl['cz']:g}, "
+                  f"no-slip z walls, gravity {wl['gz']:g}, density ratio 1000; J = centroid height after "
+                  f"{steps} steps; FP64 forward and recomputation, {state} adjoint Poisson state"),
+        steps=steps, checkpoint_interval=checkpoint, poisson_iterations_per_step=iters,
+        forward_ms=fwd_ms, adjoint_forward_ms=st.forward_ms, reverse_ms=st.reverse_ms,
+        adjoint_over_forward=st.reverse_ms / fwd_ms,
+        total_over_forward=(st.forward_ms + st.reverse_ms) / fwd_ms,
+        recomputed_iterations=int(st.recomputed_iterations), adjoint_iterations=int(st.adjoint_iterations),
+        store_slots=int(st.store_slots), J=J, dJ_dkg=gk, grad_phi0_linf=float(abs(g).max()),
+    )
+    if fd:
+        vals = []
+        for kg in (P.k_g + h, P.k_g - h):
+            x = sim_for(dataclasses.replace(P, k_g=kg))
+            vals.append(x.objective(phi0, None, iters))
+            print(f"falling adjoint leg: k_g = {kg:.6e}: J = {vals[-1]:.12e}", file=sys.stderr, flush=True)
+            x.close()
+        fdv = (vals[0] - vals[1]) / (2 * h)
+        rec["finite_difference_kg"] = dict(h=h, fd=fdv, adjoint=gk, rel_error=abs(fdv - gk) / abs(fdv))
+    return rec
+
+
 def cpu_baseline(hip, wl, workload, n_it_per_step):
     """Single-thread CPU oracle on the same initial state: one non-Poisson step
     part plus two Poisson iterations, extrapolated with the GPU's measured
@@ -287,6 +357,11 @@ def main(argv=None):
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch Poisson iterations one by one instead of replaying graphs")
     ap.add_argument("--adjoint-checkpoint", type=int, default=4)
+    ap.add_argument("--adjoint-falling", type=int, default=0, metavar="T",
+                    help="only the falling droplet's adjoint (centroid height, k_g) over T steps")
+    ap.add_argument("--adjoint-falling-state", choices=["fp32", "fp64"], default="fp32")
+    ap.add_argument("--adjoint-falling-fd", action="store_true",
+                    help="with --adjoint-falling: central differences in k_g")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one short solve per precision (for rocprofv3 --pmc runs) and exit")
     args = ap.parse_args(argv)
@@ -300,6 +375,11 @@ def main(argv=None):
     global GRAPHS
     GRAPHS = not args.no_graphs
 
+    if args.adjoint_falling:
+        rec = run_adjoint_falling(hip, args.adjoint_falling, min(200, args.adjoint_falling),
+                                  args.adjoint_falling_state, fd=args.adjoint_falling_fd)
+        print(json.dumps(rec), flush=True)
+        return 0
     if args.adjoint_only:
         rec = run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint)
         if args.adjoint_fd:

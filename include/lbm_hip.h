@@ -250,6 +250,23 @@ int lbm_adjoint(void* ctx, const double* phi0, const double* target, int32_t ste
 enum { LBM_OBJ_MISFIT = 0, LBM_OBJ_CENTROID_Z = 1 };
 int lbm_set_adjoint_objective(void* ctx, int32_t kind);
 
+/* Progress of lbm_adjoint (no counterpart in the reference): fn(user, phase,
+ * step, iterations) after each forward-pass step (its Poisson iterations),
+ * before each recorded block of the reverse sweep (the block's first
+ * iteration) and after each reversed step.  The stream is drained before each
+ * call, so a report follows the work it names.  NULL switches reports off. */
+enum { LBM_PROGRESS_FORWARD_STEP = 0, LBM_PROGRESS_REVERSE_BLOCK = 1, LBM_PROGRESS_REVERSE_STEP = 2 };
+typedef void (*lbm_progress_fn)(void* user, int32_t phase, int32_t step, uint64_t iterations);
+int lbm_set_progress(void* ctx, lbm_progress_fn fn, void* user);
+
+/* Storage of the adjoint Poisson state (no counterpart in the reference):
+ * LBM_ADJ_STATE_FP64 (the default) or LBM_ADJ_STATE_FP32, which keeps the
+ * adjoint of h between iterations in FP32 while its arithmetic, the adjoint
+ * gauge reductions, the source and tau adjoints and every other adjoint stay
+ * FP64.  The forward recomputation is FP64 either way. */
+enum { LBM_ADJ_STATE_FP64 = 0, LBM_ADJ_STATE_FP32 = 1 };
+int lbm_set_adjoint_state_precision(void* ctx, int32_t precision);
+
 /* Device memory for recorded Poisson iterates in the reverse sweep (0 =
  * automatic, 90% of what is free beyond 2 GiB).  A solve longer than the
  * recorded block is replayed in blocks from raw-h snapshots. */

@@ -104,6 +104,9 @@ struct Ctx {
     size_t sched_pos = 0;
     int64_t adj_store_bytes = 0;     // device bytes for recorded Poisson iterates (0 = automatic)
     int adj_objective = lbm::ADJ_OBJ_MISFIT;
+    bool adj_state_f32 = false;      // adjoint Poisson carriers stored in FP32
+    lbm_progress_fn progress = nullptr;  // lbm_adjoint progress reports
+    void* progress_user = nullptr;
     lbm_adjoint_stats_t adj_stats{};
 
     // Captured Poisson segments (setUsePoissonGraph): one graph per entry
@@ -688,9 +691,10 @@ void ensure_scratch(Ctx& c)
 //                  iterate and run their adjoints (kernels_adjoint.hip);
 //                  then the macro, collision and stencil adjoints.
 // The recorded iterate of iteration k is the raw streamed h the forward
-// kernel writes anyway, so recording adds no traffic.  A long solve without
-// recorded snapshots (pool exhausted) recomputes each block's start from the
-// beginning of the solve instead.
+// kernel writes anyway, so recording adds no traffic.  A solve whose
+// snapshots did not fit in the pool during the forward pass is reversed with
+// binomial (revolve-style) checkpointing over its blocks in the pool space
+// left free: with F free snapshots and d recompute passes, C(F + d, d) blocks.
 
 constexpr size_t STATE_PER_CELL = 7 + 2 * lbm::Q;  // fei rho u v w p p_prev, f, g
 
@@ -714,10 +718,12 @@ struct AdjWork {
     size_t snap_cap = 0, snap_next = 0;
     std::vector<std::vector<double*>> step_snaps;  // per step; empty = not recorded
     std::vector<char> have_snaps;
+    std::vector<size_t> pool_mark;  // per step: first pool snapshot free in its reverse
+    int lcur = 0;                   // carrier of the adjoint Poisson state in use
     double *lfei = nullptr, *lrho = nullptr, *lu = nullptr, *lv = nullptr, *lw = nullptr;
     double *lp = nullptr, *lps = nullptr, *lpf = nullptr, *lS = nullptr, *ltau = nullptr;
     double *lmac = nullptr, *pc = nullptr, *parts = nullptr, *target = nullptr;
-    double* Lb[2] = {nullptr, nullptr};
+    void* Lb[2] = {nullptr, nullptr};  // adjoint Poisson carriers, FP64 or FP32
     double* cf[2] = {nullptr, nullptr};
     double* cg[2] = {nullptr, nullptr};
     lbm::AdjState* as = nullptr;
@@ -740,6 +746,7 @@ struct AdjWork {
         if (has) (void)hipHostFree(has);
     }
     double* slot(const Ctx& c, uint64_t m) const { return slots + (m - 1) * lbm::Q * c.n(); }
+    double* pool(size_t i) const { return snap_pool + i * snap_stride; }
 };
 
 // A snapshot buffer: raw h at offset 0, gauged p at 15n, shift at 16n.
@@ -860,44 +867,166 @@ void adj_forward_block(Ctx& c, AdjWork& a, const double* p_start, double* start,
     HIPCHK(hipGetLastError());
 }
 
-// Recompute iterations 1..a1 of the current step through slots 1 and 2 and
-// leave the state of check a1 in snapshot `dst_snap` (for a solve whose
-// snapshots were not recorded).
-void adj_forward_prefix(Ctx& c, AdjWork& a, const double* p_start, double* dst_snap, uint64_t a1)
+// Recompute iterations from+1..marks.back().first of the current step through
+// slots 1 and 2, from snapshot `start` (check `from`; nullptr with from = 0:
+// the solve's start), and leave the state of each listed check in its
+// snapshot.
+void adj_forward_snaps(Ctx& c, AdjWork& a, const double* p_start, const double* start, uint64_t from,
+                       const std::vector<std::pair<uint64_t, double*>>& marks)
 {
     const uint64_t ci = static_cast<uint64_t>(c.check_interval);
     const size_t n = c.n();
-    const SnapView v = snap_view(c, dst_snap);
     int mode = lbm::H_EQUILIBRIUM;
     const double* in = a.slot(c, 1);  // not read in the equilibrium mode
     const double* plast = p_start;
-    for (uint64_t k = 1; k <= a1; ++k) {
-        double* out = k == a1 ? v.h : a.slot(c, 1 + (k & 1));
+    if (from > 0) {
+        const SnapView v = snap_view(c, const_cast<double*>(start));
+        mode = lbm::H_GAUGED;
+        in = v.h;
+        plast = v.p;
+        d2d(c, &c.dst->shift, v.shift, sizeof(double));
+    }
+    size_t m = 0;
+    for (uint64_t k = from + 1; m < marks.size(); ++k) {
+        const bool mark = k == marks[m].first;
+        const SnapView v = snap_view(c, mark ? marks[m].second : a.slot(c, 1));
+        double* out = mark ? v.h : a.slot(c, 1 + (k & 1));
         lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, plast, D<double>(c.tau), D<double>(c.src), c.dst,
                                true);
         if (k % ci == 0) {
-            double* pk = k == a1 ? v.p : a.pstore + ((k / ci) & 1) * n;
+            double* pk = mark ? v.p : a.pstore + ((k / ci) & 1) * n;
             lbm::f64::gauge(c.stream, c.g, c.ph, out, pk, c.p_prev, c.partials, c.dst);
-            if (k == a1) d2d(c, v.shift, &c.dst->shift, sizeof(double));
+            if (mark) d2d(c, v.shift, &c.dst->shift, sizeof(double));
             plast = pk;
             mode = lbm::H_GAUGED;
         } else {
             mode = lbm::H_STREAMED;
         }
         in = out;
+        if (mark) ++m;
     }
     HIPCHK(hipGetLastError());
 }
 
-// Reverse of one step whose start state is st (already restored into c).
-// snaps: the forward pass's snapshots of this solve, if it recorded them.
-void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*>& snaps, bool have_snaps,
-              uint64_t nit, int ccur, lbm_adjoint_stats_t& stats)
+// C(f + d, d), saturating: the blocks d recompute passes reverse with f free
+// snapshots.
+uint64_t revolve_capacity(uint64_t f, uint64_t d)
+{
+    double r = 1.0;
+    for (uint64_t i = 1; i <= d; ++i) r = r * static_cast<double>(f + i) / static_cast<double>(i);
+    return r > 1e18 ? static_cast<uint64_t>(1e18) : static_cast<uint64_t>(r + 0.5);
+}
+
+// Progress report of lbm_adjoint: the stream is drained first, so the report
+// follows the work it names.
+void report(Ctx& c, int phase, int step, uint64_t iterations)
+{
+    if (!c.progress) return;
+    sync(c);
+    c.progress(c.progress_user, phase, step, iterations);
+}
+
+// One step's reverse: the context of adj_block.
+struct StepRev {
+    const double* p_start;
+    uint64_t nit;
+    int step;
+    lbm_adjoint_stats_t* stats;
+};
+
+// Block j of the solve (iterations jL+1 .. min(nit, (j+1)L)) from snapshot
+// `start` of check jL (nullptr for j = 0): recompute it recording every
+// iterate, then run the adjoints of its iterations, last first.
+void adj_block(Ctx& c, AdjWork& a, const StepRev& r, uint64_t j, double* start)
 {
     hipStream_t s = c.stream;
     const Grid& g = c.g;
     const size_t n = c.n();
     const uint64_t ci = static_cast<uint64_t>(c.check_interval);
+    const uint64_t nit = r.nit;
+    const uint64_t a0 = j * a.L, a1 = std::min(nit, a0 + a.L);
+    report(c, LBM_PROGRESS_REVERSE_BLOCK, r.step, a0);
+    adj_forward_block(c, a, r.p_start, start, a0, a1);
+    r.stats->recomputed_iterations += a1 - a0;
+    if (a1 == nit) {
+        // correct_uvw reads the solve's final (gauged) p.
+        const double* p_final = a.pstore + ((nit - a0) / ci) * n;
+        lbm::adj::correct(s, g, c.ph, p_final, D<double>(c.rho), a.lu, a.lv, a.lw, a.lp, a.lrho, a.lpf,
+                          a.parts, a.as);
+    }
+    const SnapView sv = start ? snap_view(c, start) : SnapView{nullptr, nullptr, nullptr};
+    for (uint64_t k = a1; k > a0; --k) {
+        const int src = k == nit ? lbm::ADJ_SRC_INIT : (k % ci == 0 ? lbm::ADJ_SRC_LAZY : lbm::ADJ_SRC_PLAIN);
+        const int mode = k == 1 ? lbm::H_EQUILIBRIUM
+                                : ((k - 1) % ci == 0 ? lbm::H_GAUGED : lbm::H_STREAMED);
+        const double* raw = k >= a0 + 2 ? a.slot(c, k - 1 - a0) : sv.h;
+        const double* pin = nullptr;
+        const double* shift = nullptr;
+        if (mode == lbm::H_EQUILIBRIUM) {
+            pin = r.p_start;
+        } else if (mode == lbm::H_GAUGED) {
+            pin = k - 1 == a0 ? sv.p : a.pstore + ((k - 1 - a0) / ci) * n;
+            shift = k - 1 == a0 ? sv.shift : a.shifts + (k - 1 - a0) / ci;
+        }
+        lbm::adj::poisson(s, g, c.ph, src, mode, c.adj_state_f32, a.Lb[a.lcur], a.lpf, a.as, raw, pin,
+                          shift, D<double>(c.tau), a.lS, a.ltau, k == 1 ? nullptr : a.Lb[1 - a.lcur],
+                          k == 1 ? a.lps : nullptr, a.parts);
+        if (k > 1) a.lcur = 1 - a.lcur;
+    }
+    HIPCHK(hipGetLastError());
+}
+
+// Reverse blocks [lo, hi) from snapshot `start` of block lo's start, with
+// pool snapshots fb.. free: split into segments whose lengths are the
+// capacities C(F - i + d - 1, d - 1) of one pass fewer, snapshot their starts
+// in one forward pass, and reverse the segments last first, segment i with
+// the snapshots after its own free.  No free snapshot: each block's start is
+// recomputed from `start` into the scratch snapshot.
+void adj_reverse_range(Ctx& c, AdjWork& a, const StepRev& r, uint64_t lo, uint64_t hi, double* start,
+                       size_t fb)
+{
+    const uint64_t L = a.L;
+    if (hi - lo == 1) {
+        adj_block(c, a, r, lo, start);
+        return;
+    }
+    const uint64_t F = fb < a.snap_cap ? a.snap_cap - fb : 0;
+    if (F == 0) {
+        for (uint64_t j = hi; j-- > lo;) {
+            double* sj = start;
+            if (j > lo) {
+                adj_forward_snaps(c, a, r.p_start, start, lo * L, {{j * L, a.scratch_snap}});
+                r.stats->recomputed_iterations += (j - lo) * L;
+                sj = a.scratch_snap;
+            }
+            adj_block(c, a, r, j, sj);
+        }
+        return;
+    }
+    uint64_t d = 1;
+    while (revolve_capacity(F, d) < hi - lo) ++d;
+    std::vector<uint64_t> b{lo};
+    for (uint64_t i = 0; b.back() < hi; ++i) {
+        const uint64_t len = i <= F ? revolve_capacity(F - i, d - 1) : hi - b.back();
+        b.push_back(std::min(hi, b.back() + len));
+    }
+    const size_t segs = b.size() - 1;
+    std::vector<std::pair<uint64_t, double*>> marks;
+    for (size_t i = 1; i < segs; ++i) marks.emplace_back(b[i] * L, a.pool(fb + i - 1));
+    adj_forward_snaps(c, a, r.p_start, start, lo * L, marks);
+    r.stats->recomputed_iterations += (b[segs - 1] - lo) * L;
+    for (size_t i = segs; i-- > 0;)
+        adj_reverse_range(c, a, r, b[i], b[i + 1], i == 0 ? start : a.pool(fb + i - 1), fb + i);
+}
+
+// Reverse of one step whose start state is st (already restored into c).
+// snaps: the forward pass's snapshots of this solve, if it recorded them.
+void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*>& snaps, bool have_snaps,
+              size_t pool_free, int step, uint64_t nit, int ccur, lbm_adjoint_stats_t& stats)
+{
+    hipStream_t s = c.stream;
+    const Grid& g = c.g;
+    const size_t n = c.n();
     const uint64_t L = a.L;
     const double* s_fei = st.buf;
     const double* s_rho = st.buf + n;
@@ -911,47 +1040,12 @@ void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*
     const uint64_t nblocks = (nit + L - 1) / L;
     HIPCHK(hipMemsetAsync(a.lS, 0, n * sizeof(double), s));
     HIPCHK(hipMemsetAsync(a.ltau, 0, n * sizeof(double), s));
-    int lcur = 0;
-    for (uint64_t j = nblocks; j-- > 0;) {
-        const uint64_t a0 = j * L, a1 = std::min(nit, a0 + L);
-        double* start = nullptr;
-        if (j > 0) {
-            if (have_snaps) {
-                start = snaps[j - 1];
-            } else {
-                adj_forward_prefix(c, a, p_start, a.scratch_snap, a0);
-                stats.recomputed_iterations += a0;
-                start = a.scratch_snap;
-            }
-        }
-        adj_forward_block(c, a, p_start, start, a0, a1);
-        stats.recomputed_iterations += a1 - a0;
-        if (j == nblocks - 1) {
-            // correct_uvw reads the solve's final (gauged) p.
-            const double* p_final = a.pstore + ((nit - a0) / ci) * n;
-            lbm::adj::correct(s, g, c.ph, p_final, D<double>(c.rho), a.lu, a.lv, a.lw, a.lp, a.lrho, a.lpf,
-                              a.parts, a.as);
-        }
-        const SnapView sv = start ? snap_view(c, start) : SnapView{nullptr, nullptr, nullptr};
-        for (uint64_t k = a1; k > a0; --k) {
-            const int src = k == nit ? lbm::ADJ_SRC_INIT : (k % ci == 0 ? lbm::ADJ_SRC_LAZY : lbm::ADJ_SRC_PLAIN);
-            const int mode = k == 1 ? lbm::H_EQUILIBRIUM
-                                    : ((k - 1) % ci == 0 ? lbm::H_GAUGED : lbm::H_STREAMED);
-            const double* raw = k >= a0 + 2 ? a.slot(c, k - 1 - a0) : sv.h;
-            const double* pin = nullptr;
-            const double* shift = nullptr;
-            if (mode == lbm::H_EQUILIBRIUM) {
-                pin = p_start;
-            } else if (mode == lbm::H_GAUGED) {
-                pin = k - 1 == a0 ? sv.p : a.pstore + ((k - 1 - a0) / ci) * n;
-                shift = k - 1 == a0 ? sv.shift : a.shifts + (k - 1 - a0) / ci;
-            }
-            double* out = k == 1 ? a.lps : a.Lb[1 - lcur];
-            lbm::adj::poisson(s, g, c.ph, src, mode, a.Lb[lcur], a.lpf, a.as, raw, pin, shift,
-                              D<double>(c.tau), a.lS, a.ltau, out, a.parts);
-            if (k > 1) lcur = 1 - lcur;
-        }
-        HIPCHK(hipGetLastError());
+    a.lcur = 0;
+    const StepRev r{p_start, nit, step, &stats};
+    if (have_snaps || nblocks == 1) {
+        for (uint64_t j = nblocks; j-- > 0;) adj_block(c, a, r, j, j > 0 ? snaps[j - 1] : nullptr);
+    } else {
+        adj_reverse_range(c, a, r, 0, nblocks, nullptr, pool_free);
     }
     stats.adjoint_iterations += nit;
     lbm::adj::source(s, g, c.ph, D<double>(c.fei), D<double>(c.rho), a.ltau, a.lS, a.lfei, a.as, a.lrho,
@@ -998,7 +1092,7 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     for (double** p : {&a.lfei, &a.lrho, &a.lu, &a.lv, &a.lw, &a.lp, &a.lps, &a.lpf, &a.lS, &a.ltau, &a.lmac, &a.target})
         *p = dalloc(n);
     for (int i = 0; i < 2; ++i) {
-        a.Lb[i] = dalloc(lbm::Q * n);
+        a.Lb[i] = dalloc_bytes(lbm::Q * n * (c.adj_state_f32 ? sizeof(float) : sizeof(double)));
         a.cf[i] = dalloc(lbm::Q * n);
         a.cg[i] = dalloc(lbm::Q * n);
     }
@@ -1034,6 +1128,7 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     if (a.snap_cap) a.snap_pool = dalloc(a.snap_cap * a.snap_stride);
     a.step_snaps.resize(static_cast<size_t>(T));
     a.have_snaps.assign(static_cast<size_t>(T), 0);
+    a.pool_mark.assign(static_cast<size_t>(T), 0);
 
     // Initial state: rho0 linear in phi0 (place_droplet's rule), rest zero.
     HostFields h0;
@@ -1058,10 +1153,12 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
         const size_t ks = static_cast<size_t>(k);
         if (k % K == 0) save_state(c, a.ckpt[static_cast<size_t>(k / K)]);
         const size_t pool_mark = a.snap_next;
+        a.pool_mark[ks] = pool_mark;
         SnapRecorder rec{&a, &a.step_snaps[ks]};
         const CheckHook hook{record_snapshot, &rec};
         step_one(c, &hook);
         const uint64_t nit = c.history.back();
+        report(c, LBM_PROGRESS_FORWARD_STEP, k, nit);
         const size_t need = static_cast<size_t>((nit + L - 1) / L - 1);
         if (rec.overflow) {
             a.step_snaps[ks].clear();  // incomplete: this step takes its own snapshot pass
@@ -1105,7 +1202,9 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
             const size_t ks = static_cast<size_t>(k);
             const StepState& st = a.seg[static_cast<size_t>(k - k0)];
             restore_state(c, st);
-            adj_step(c, a, st, a.step_snaps[ks], a.have_snaps[ks] != 0, sched[ks], ccur, stats);
+            adj_step(c, a, st, a.step_snaps[ks], a.have_snaps[ks] != 0, a.pool_mark[ks], k, sched[ks], ccur,
+                     stats);
+            report(c, LBM_PROGRESS_REVERSE_STEP, k, sched[ks]);
             ccur = 1 - ccur;
         }
     }
@@ -1488,6 +1587,23 @@ int lbm_set_adjoint_objective(void* ctx, int32_t kind)
         if (kind != LBM_OBJ_MISFIT && kind != LBM_OBJ_CENTROID_Z)
             throw ApiError{LBM_ERR_ARG, "unknown adjoint objective"};
         c.adj_objective = kind == LBM_OBJ_CENTROID_Z ? lbm::ADJ_OBJ_CENTROID_Z : lbm::ADJ_OBJ_MISFIT;
+    });
+}
+
+int lbm_set_progress(void* ctx, lbm_progress_fn fn, void* user)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        c.progress = fn;
+        c.progress_user = user;
+    });
+}
+
+int lbm_set_adjoint_state_precision(void* ctx, int32_t precision)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        if (precision != LBM_ADJ_STATE_FP64 && precision != LBM_ADJ_STATE_FP32)
+            throw ApiError{LBM_ERR_ARG, "unknown adjoint state precision"};
+        c.adj_state_f32 = precision == LBM_ADJ_STATE_FP32;
     });
 }
 

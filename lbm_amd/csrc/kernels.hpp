@@ -129,10 +129,13 @@ void seed(hipStream_t s, const Grid& g, const Phys& ph, int objective, const dou
 void correct(hipStream_t s, const Grid& g, const Phys& ph, const double* p, const double* rho,
              const double* lu, const double* lv, const double* lw, const double* lp, double* lrho,
              double* lpf, double* parts, AdjState* as);
-void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, const double* Bin,
-             const double* lpf, AdjState* as, const double* raw, const double* pin,
-             const double* shift, const double* tau, double* lS, double* ltau, double* Bout,
-             double* parts);
+// Bin / Bout are [q][n] carriers of the adjoint, float when state_f32 is set
+// and double otherwise; the first iteration (mode H_EQUILIBRIUM) writes
+// lp_start instead of Bout.
+void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, bool state_f32,
+             const void* Bin, const double* lpf, AdjState* as, const double* raw, const double* pin,
+             const double* shift, const double* tau, double* lS, double* ltau, void* Bout,
+             double* lp_start, double* parts);
 void source(hipStream_t s, const Grid& g, const Phys& ph, const double* fei, const double* rho,
             const double* ltau, const double* lS, const double* lfei, AdjState* as, double* lrho,
             double* lps, double* lmac, double* parts);

@@ -290,7 +290,7 @@ enum { SRC_PLAIN = 0, SRC_LAZY = 1, SRC_INIT = 2 };
 // value the slip overwrote, 2 where it also fed its mirror partner).  For the
 // last iteration (SRC_INIT) every viewed value at y carries lpf(y), gathered.
 // MODE is how forward iteration k obtained its input (h_{k-1}, p_{k-1}):
-//   H_EQUILIBRIUM: h = E p_start (first iteration); writes Bout = dJ/dp_start
+//   H_EQUILIBRIUM: h = E p_start (first iteration); writes lp_start = dJ/dp_start
 //   H_GAUGED:      h = view(raw) - E shift, p = gauged p of the check; block
 //                  partials of sum_q E_q L_q for that check's adjoint
 //   H_STREAMED:    h = view(raw), p = sum_q h_q
@@ -299,13 +299,16 @@ enum { SRC_PLAIN = 0, SRC_LAZY = 1, SRC_INIT = 2 };
 //   ltau += sum lh* (h - E p) / tau^2,
 // and L = lv + lp is the adjoint of iteration k-1's viewed output; its slip
 // transpose at x is pushed to x - c_q for the next (earlier) iteration.
-template <int SRC, int MODE>
+// TB is the storage type of the carried adjoint Bin/Bout: double, or float
+// for the FP32 adjoint state (arithmetic, lS, ltau and the gauge partials stay
+// FP64 either way).
+template <int SRC, int MODE, typename TB>
 __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
-    Grid g, const double* __restrict__ Bin, const double* __restrict__ lpf,
+    Grid g, const TB* __restrict__ Bin, const double* __restrict__ lpf,
     const AdjState* __restrict__ as, const double* __restrict__ raw,
     const double* __restrict__ pin, const double* __restrict__ shift_ptr,
     const double* __restrict__ tau, double* __restrict__ lS, double* __restrict__ ltau,
-    double* __restrict__ Bout, double* __restrict__ parts)
+    TB* __restrict__ Bout, double* __restrict__ lp_start, double* __restrict__ parts)
 {
     __shared__ double lds[4];
     const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
@@ -319,7 +322,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
             gather_stream_slip_T(x, y, z, g, [&](int, int64_t d) { return lpf[d] - ls; }, lh);
         } else {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) lh[q] = __builtin_nontemporal_load(&Bin[q * g.n + c]);
+            for (int q = 0; q < Q; ++q) lh[q] = static_cast<double>(__builtin_nontemporal_load(&Bin[q * g.n + c]));
             if (SRC == SRC_LAZY) {
                 const double ls = as->lam_shift;
 #pragma unroll
@@ -359,7 +362,7 @@ __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
             double b = np;
 #pragma unroll
             for (int q = 0; q < Q; ++q) b += EW[q] * lh[q];
-            Bout[c] = b;
+            lp_start[c] = b;
         } else {
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
@@ -390,12 +393,12 @@ __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
                 // the streamed value q at x came from x - c_q, or, across a
                 // z wall, from this cell's opposite population
                 if (z_bounced_in(q, z, g)) {
-                    Bout[OPP[q] * g.n + c] = lh[q];
+                    Bout[OPP[q] * g.n + c] = static_cast<TB>(lh[q]);
                 } else {
                     const int sx = CX[q] == 1 ? xm : (CX[q] == -1 ? xp : x);
                     const int sy = CY[q] == 1 ? ym : (CY[q] == -1 ? yp : y);
                     const int sz = CZ[q] == 1 ? zm : (CZ[q] == -1 ? zp : z);
-                    Bout[q * g.n + lin(sx, sy, sz, g)] = lh[q];
+                    Bout[q * g.n + lin(sx, sy, sz, g)] = static_cast<TB>(lh[q]);
                 }
             }
         }
@@ -717,15 +720,16 @@ void correct(hipStream_t s, const Grid& g, const Phys& ph, const double* p, cons
     k_adj_lam_finalize<<<1, BLOCK, 0, s>>>(parts, nb, ph.cell_count, 1, as);
 }
 
-void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, const double* Bin,
-             const double* lpf, AdjState* as, const double* raw, const double* pin,
-             const double* shift, const double* tau, double* lS, double* ltau, double* Bout,
-             double* parts)
+template <typename TB>
+static void poisson_t(hipStream_t s, const Grid& g, int src, int mode, const TB* Bin,
+                      const double* lpf, AdjState* as, const double* raw, const double* pin,
+                      const double* shift, const double* tau, double* lS, double* ltau, TB* Bout,
+                      double* lp_start, double* parts)
 {
     const int nb = grid_blocks(g.n);
-#define LBM_ADJ_LAUNCH(S, M)                                                                     \
-    k_adj_poisson<S, M><<<nb, BLOCK, 0, s>>>(g, Bin, lpf, as, raw, pin, shift, tau, lS, ltau, Bout, \
-                                             parts)
+#define LBM_ADJ_LAUNCH(S, M)                                                                       \
+    k_adj_poisson<S, M, TB><<<nb, BLOCK, 0, s>>>(g, Bin, lpf, as, raw, pin, shift, tau, lS, ltau, Bout, \
+                                                 lp_start, parts)
 #define LBM_ADJ_MODES(S)                                         \
     do {                                                         \
         if (mode == H_EQUILIBRIUM) LBM_ADJ_LAUNCH(S, H_EQUILIBRIUM); \
@@ -737,7 +741,20 @@ void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, co
     else LBM_ADJ_MODES(SRC_PLAIN);
 #undef LBM_ADJ_MODES
 #undef LBM_ADJ_LAUNCH
-    if (mode == H_GAUGED) k_adj_lam_finalize<<<1, BLOCK, 0, s>>>(parts, nb, ph.cell_count, 0, as);
+}
+
+void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, bool state_f32,
+             const void* Bin, const double* lpf, AdjState* as, const double* raw, const double* pin,
+             const double* shift, const double* tau, double* lS, double* ltau, void* Bout,
+             double* lp_start, double* parts)
+{
+    if (state_f32)
+        poisson_t(s, g, src, mode, static_cast<const float*>(Bin), lpf, as, raw, pin, shift, tau, lS,
+                  ltau, static_cast<float*>(Bout), lp_start, parts);
+    else
+        poisson_t(s, g, src, mode, static_cast<const double*>(Bin), lpf, as, raw, pin, shift, tau, lS,
+                  ltau, static_cast<double*>(Bout), lp_start, parts);
+    if (mode == H_GAUGED) k_adj_lam_finalize<<<1, BLOCK, 0, s>>>(parts, grid_blocks(g.n), ph.cell_count, 0, as);
 }
 
 void source(hipStream_t s, const Grid& g, const Phys& ph, const double* fei, const double* rho,

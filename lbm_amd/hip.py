@@ -44,7 +44,12 @@ EXPORTED = (
     "lbm_reset_fields", "lbm_set_iteration_schedule", "lbm_iteration_history",
     "lbm_adjoint", "lbm_set_adjoint_memory", "lbm_adjoint_stats", "lbm_set_poisson_graph",
     "lbm_set_z_walls", "lbm_set_gravity", "lbm_set_adjoint_objective",
+    "lbm_set_adjoint_state_precision", "lbm_set_progress",
 )
+
+# lbm_progress_fn: (user, phase, step, iterations)
+PROGRESS_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int32, ctypes.c_int32, ctypes.c_uint64)
+PROGRESS_PHASES = {0: "forward step", 1: "reverse block", 2: "reverse step"}
 
 # Objectives of lbm_set_adjoint_objective.
 OBJECTIVES = {"misfit": 0, "centroid_z": 1}
@@ -139,6 +144,8 @@ def load_library():
     L.lbm_set_z_walls.argtypes = [vp, ctypes.c_int]
     L.lbm_set_gravity.argtypes = [vp, ctypes.c_double]
     L.lbm_set_adjoint_objective.argtypes = [vp, ctypes.c_int32]
+    L.lbm_set_adjoint_state_precision.argtypes = [vp, ctypes.c_int32]
+    L.lbm_set_progress.argtypes = [vp, PROGRESS_FN, vp]
     L.lbm_adjoint_stats.argtypes = [vp, ctypes.POINTER(AdjointStats)]
     _lib = L
     return L
@@ -371,6 +378,22 @@ class InamuroHIP:
         height; no target)."""
         self._check(load_library().lbm_set_adjoint_objective(self._h, OBJECTIVES[kind]))
         self._objective = kind
+
+    def set_progress(self, fn):
+        """fn(phase, step, iterations) with phase 'forward step' / 'reverse
+        block' / 'reverse step', called by lbm_adjoint as its work finishes;
+        None switches reports off."""
+        if fn is None:
+            self._progress = None
+            self._check(load_library().lbm_set_progress(self._h, PROGRESS_FN(), None))
+            return
+        self._progress = PROGRESS_FN(lambda _u, ph, st, it: fn(PROGRESS_PHASES.get(ph, str(ph)), st, it))
+        self._check(load_library().lbm_set_progress(self._h, self._progress, None))
+
+    def set_adjoint_state_precision(self, precision: str):
+        """'fp64' (default) or 'fp32': storage of the adjoint Poisson state
+        between iterations (arithmetic and reductions stay FP64)."""
+        self._check(load_library().lbm_set_adjoint_state_precision(self._h, {"fp64": 0, "fp32": 1}[precision]))
 
     def adjoint(self, phi0, target, steps: int, checkpoint_interval: int = 100, iterations=None):
         """(J, dJ/dphi0, dJ/dk_g) of the objective (set_adjoint_objective)

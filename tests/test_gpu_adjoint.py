@@ -88,15 +88,36 @@ def test_recomputation_does_not_change_the_result():
         # and the block's check store; a quarter is the snapshot pool
         return int(8 * 4 / 3 * (L + 1) * (15 * n + n // 10 + 1))
 
-    # L ~ 20: the pool holds too few snapshots, every long solve recomputes
-    # its block starts from the beginning; L ~ 50: all snapshots recorded.
-    for K, L in ((1, 0), (2, 0), (4, 20), (3, 20), (4, 50)):
+    # L ~ 10, 20: the pool holds too few snapshots for the long solves, which
+    # are reversed by binomial checkpointing over their blocks (several
+    # recompute passes); L ~ 50: all snapshots recorded in the forward pass.
+    for K, L in ((1, 0), (2, 0), (4, 20), (3, 20), (4, 10), (1, 10), (4, 50)):
         sim.set_adjoint_memory(budget(L) if L else 0)
         J, g, k = sim.adjoint(phi0, target, 4, checkpoint_interval=K)
         st = sim.adjoint_stats()
         if L:
             assert st.store_slots < m and st.recomputed_iterations >= sum(iters), (st.store_slots, m)
         assert J == J0 and k == k0 and np.array_equal(g, g0), (K, L)
+
+
+def test_fp32_state_matches_fp64_state():
+    """The adjoint Poisson state stored in FP32 (arithmetic and reductions in
+    FP64): density ratio 1000 with thousands of iterations per step, and the
+    walls + gravity centroid case.  Measured differences are ~1e-8."""
+    announce("adjoint FP32 state vs FP64 state")
+    P = params(lx=10, ly=8, lz=8, rho_L=1000.0)
+    phi0, target, _ = case(P)
+    res = {}
+    for prec in ("fp64", "fp32"):
+        sim = hip.InamuroHIP(P)
+        sim.setPoisson(check_interval=100)
+        sim.set_adjoint_state_precision(prec)
+        res[prec] = sim.adjoint(phi0, target, 3, checkpoint_interval=2)
+        assert sim.iteration_history()[-1] >= 1000
+    (J64, g64, k64), (J32, g32, k32) = res["fp64"], res["fp32"]
+    assert J32 == J64
+    assert rel(g32, g64) <= 1e-6, rel(g32, g64)
+    assert abs(k32 - k64) <= 1e-6 * abs(k64), (k32, k64)
 
 
 def test_context_state_after_adjoint_is_the_forward_end():
