@@ -717,7 +717,7 @@ struct AdjWork {
     size_t snap_stride = 0;       // doubles per snapshot
     size_t snap_cap = 0, snap_next = 0;
     std::vector<std::vector<double*>> step_snaps;  // per step; empty = not recorded
-    std::vector<char> have_snaps;
+    std::vector<char> have_snaps;   // SNAPS_*
     std::vector<size_t> pool_mark;  // per step: first pool snapshot free in its reverse
     int lcur = 0;                   // carrier of the adjoint Poisson state in use
     double *lfei = nullptr, *lrho = nullptr, *lu = nullptr, *lv = nullptr, *lw = nullptr;
@@ -793,16 +793,26 @@ void restore_state(Ctx& c, const StepState& st)
 
 // Forward-pass hook: after the check at iteration L*j keep raw h, the gauged
 // p and the shift in the step's next snapshot, while the pool has room.
+// With `marks` (a solve of known length whose block snapshots do not all fit)
+// only the checks listed there are kept: the top level of its binomial
+// checkpointing.
 struct SnapRecorder {
     AdjWork* a;
     std::vector<double*>* snaps;
     bool overflow = false;
+    std::vector<uint64_t> marks;
+    size_t next = 0;
 };
 
 void record_snapshot(Ctx& c, uint64_t iter, void* arg)
 {
     SnapRecorder& r = *static_cast<SnapRecorder*>(arg);
-    if (iter % r.a->L != 0 || r.overflow) return;
+    if (!r.marks.empty()) {
+        if (r.next >= r.marks.size() || iter != r.marks[r.next]) return;
+        ++r.next;
+    } else if (iter % r.a->L != 0 || r.overflow) {
+        return;
+    }
     if (r.a->snap_next >= r.a->snap_cap) {
         r.overflow = true;
         return;
@@ -926,6 +936,10 @@ void report(Ctx& c, int phase, int step, uint64_t iterations)
     c.progress(c.progress_user, phase, step, iterations);
 }
 
+// What the forward pass kept of a solve: nothing, a snapshot at every block
+// boundary, or the top level of its binomial checkpointing.
+enum : char { SNAPS_NONE = 0, SNAPS_ALL = 1, SNAPS_TOP = 2 };
+
 // One step's reverse: the context of adj_block.
 struct StepRev {
     const double* p_start;
@@ -976,12 +990,38 @@ void adj_block(Ctx& c, AdjWork& a, const StepRev& r, uint64_t j, double* start)
     HIPCHK(hipGetLastError());
 }
 
+// Segment boundaries b[0] = lo < ... < b[s] = hi of blocks [lo, hi) with F
+// free snapshots: the fewest passes d with C(F + d, d) >= hi - lo, then
+// segment i of length C(F - i + d - 1, d - 1) (what d - 1 passes reverse with
+// the F - i snapshots left to it), greedily from the front.
+std::vector<uint64_t> revolve_plan(uint64_t lo, uint64_t hi, uint64_t F)
+{
+    uint64_t d = 1;
+    while (revolve_capacity(F, d) < hi - lo) ++d;
+    std::vector<uint64_t> b{lo};
+    for (uint64_t i = 0; b.back() < hi; ++i) {
+        const uint64_t len = i <= F ? revolve_capacity(F - i, d - 1) : hi - b.back();
+        b.push_back(std::min(hi, b.back() + len));
+    }
+    return b;
+}
+
+void adj_reverse_range(Ctx& c, AdjWork& a, const StepRev& r, uint64_t lo, uint64_t hi, double* start,
+                       size_t fb);
+
+// The segments of plan b, whose starts b[1..] sit in pool snapshots fb..,
+// last first; segment i reuses the snapshots after its own.
+void adj_reverse_segments(Ctx& c, AdjWork& a, const StepRev& r, const std::vector<uint64_t>& b, double* start,
+                          size_t fb)
+{
+    for (size_t i = b.size() - 1; i-- > 0;)
+        adj_reverse_range(c, a, r, b[i], b[i + 1], i == 0 ? start : a.pool(fb + i - 1), fb + i);
+}
+
 // Reverse blocks [lo, hi) from snapshot `start` of block lo's start, with
-// pool snapshots fb.. free: split into segments whose lengths are the
-// capacities C(F - i + d - 1, d - 1) of one pass fewer, snapshot their starts
-// in one forward pass, and reverse the segments last first, segment i with
-// the snapshots after its own free.  No free snapshot: each block's start is
-// recomputed from `start` into the scratch snapshot.
+// pool snapshots fb.. free: snapshot the starts of revolve_plan's segments in
+// one forward pass, then reverse the segments.  No free snapshot: each
+// block's start is recomputed from `start` into the scratch snapshot.
 void adj_reverse_range(Ctx& c, AdjWork& a, const StepRev& r, uint64_t lo, uint64_t hi, double* start,
                        size_t fb)
 {
@@ -1003,25 +1043,18 @@ void adj_reverse_range(Ctx& c, AdjWork& a, const StepRev& r, uint64_t lo, uint64
         }
         return;
     }
-    uint64_t d = 1;
-    while (revolve_capacity(F, d) < hi - lo) ++d;
-    std::vector<uint64_t> b{lo};
-    for (uint64_t i = 0; b.back() < hi; ++i) {
-        const uint64_t len = i <= F ? revolve_capacity(F - i, d - 1) : hi - b.back();
-        b.push_back(std::min(hi, b.back() + len));
-    }
+    const std::vector<uint64_t> b = revolve_plan(lo, hi, F);
     const size_t segs = b.size() - 1;
     std::vector<std::pair<uint64_t, double*>> marks;
     for (size_t i = 1; i < segs; ++i) marks.emplace_back(b[i] * L, a.pool(fb + i - 1));
     adj_forward_snaps(c, a, r.p_start, start, lo * L, marks);
     r.stats->recomputed_iterations += (b[segs - 1] - lo) * L;
-    for (size_t i = segs; i-- > 0;)
-        adj_reverse_range(c, a, r, b[i], b[i + 1], i == 0 ? start : a.pool(fb + i - 1), fb + i);
+    adj_reverse_segments(c, a, r, b, start, fb);
 }
 
 // Reverse of one step whose start state is st (already restored into c).
 // snaps: the forward pass's snapshots of this solve, if it recorded them.
-void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*>& snaps, bool have_snaps,
+void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*>& snaps, int have_snaps,
               size_t pool_free, int step, uint64_t nit, int ccur, lbm_adjoint_stats_t& stats)
 {
     hipStream_t s = c.stream;
@@ -1042,8 +1075,10 @@ void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*
     HIPCHK(hipMemsetAsync(a.ltau, 0, n * sizeof(double), s));
     a.lcur = 0;
     const StepRev r{p_start, nit, step, &stats};
-    if (have_snaps || nblocks == 1) {
+    if (have_snaps == SNAPS_ALL || nblocks == 1) {
         for (uint64_t j = nblocks; j-- > 0;) adj_block(c, a, r, j, j > 0 ? snaps[j - 1] : nullptr);
+    } else if (have_snaps == SNAPS_TOP) {
+        adj_reverse_segments(c, a, r, revolve_plan(0, nblocks, a.snap_cap - pool_free), nullptr, pool_free);
     } else {
         adj_reverse_range(c, a, r, 0, nblocks, nullptr, pool_free);
     }
@@ -1112,11 +1147,36 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     }
     const size_t slot_d = lbm::Q * n;
     a.snap_stride = ((lbm::Q + 1) * n + 1 + 31) / 32 * 32;
-    const size_t pool_d = budget / 4 / sizeof(double);
-    const size_t slots_d = budget / sizeof(double) - pool_d;
+    const size_t total_d = budget / sizeof(double);
+    auto block_mem = [&](uint64_t l) { return std::max<uint64_t>(l, 2) * slot_d + (l / ci + 1) * (n + 1); };
+    size_t pool_d = total_d / 4;
+    size_t slots_d = total_d - pool_d;
     uint64_t L = (static_cast<uint64_t>(slots_d / (slot_d + n / ci + 1)) / ci) * ci;
-    if (sched_max) L = std::min(L, sched_max);
-    while (L >= ci && std::max<uint64_t>(L, 2) * slot_d + (L / ci + 1) * (n + 1) > slots_d) L -= ci;
+    if (sched_max) {
+        // Known solve lengths: the block length whose split of the budget
+        // reverses the longest solve in the fewest recompute passes (the
+        // longest block among equals).
+        uint64_t best = 0, best_passes = ~uint64_t(0);
+        for (uint64_t l = ci; l < sched_max + ci; l += ci) {
+            const size_t mem = block_mem(l);
+            if (mem + 2 * a.snap_stride > total_d) break;
+            const uint64_t F = (total_d - mem - a.snap_stride) / a.snap_stride;
+            const uint64_t nb = (sched_max + l - 1) / l;
+            uint64_t passes = 1;
+            if (nb >= 2 && nb - 1 > F) {
+                passes = F == 0 ? nb : 1;
+                while (F > 0 && revolve_capacity(F, passes) < nb) ++passes;
+            }
+            if (passes <= best_passes) {
+                best_passes = passes;
+                best = l;
+            }
+        }
+        L = best;
+        slots_d = L ? block_mem(L) : 0;
+        pool_d = total_d - slots_d;
+    }
+    while (L >= ci && block_mem(L) > slots_d) L -= ci;
     if (L < ci) throw ApiError{LBM_ERR_NOMEM, "adjoint needs device memory for check_interval recorded Poisson iterates"};
     a.L = L;
     stats.store_slots = static_cast<int64_t>(L);
@@ -1155,18 +1215,31 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
         const size_t pool_mark = a.snap_next;
         a.pool_mark[ks] = pool_mark;
         SnapRecorder rec{&a, &a.step_snaps[ks]};
+        if (iters) {
+            // a scheduled solve whose block snapshots do not fit keeps the
+            // top level of its binomial checkpointing instead
+            const uint64_t nb = (iters[k] + L - 1) / L;
+            const uint64_t F = a.snap_cap - pool_mark;
+            if (F > 0 && nb >= 2 && nb - 1 > F) {
+                const std::vector<uint64_t> b = revolve_plan(0, nb, F);
+                for (size_t i = 1; i + 1 < b.size(); ++i) rec.marks.push_back(b[i] * L);
+            }
+        }
         const CheckHook hook{record_snapshot, &rec};
         step_one(c, &hook);
         const uint64_t nit = c.history.back();
         report(c, LBM_PROGRESS_FORWARD_STEP, k, nit);
         const size_t need = static_cast<size_t>((nit + L - 1) / L - 1);
-        if (rec.overflow) {
+        if (!rec.marks.empty()) {
+            a.snap_next = pool_mark + rec.marks.size();
+            a.have_snaps[ks] = SNAPS_TOP;
+        } else if (rec.overflow) {
             a.step_snaps[ks].clear();  // incomplete: this step takes its own snapshot pass
             a.snap_next = pool_mark;
         } else {
             a.step_snaps[ks].resize(need);  // drop a snapshot taken at the final check
             a.snap_next = pool_mark + need;
-            a.have_snaps[ks] = 1;
+            a.have_snaps[ks] = SNAPS_ALL;
         }
         d2d(c, a.pfinal + ks * n, c.p[c.pc], n * sizeof(double));
     }
@@ -1202,8 +1275,7 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
             const size_t ks = static_cast<size_t>(k);
             const StepState& st = a.seg[static_cast<size_t>(k - k0)];
             restore_state(c, st);
-            adj_step(c, a, st, a.step_snaps[ks], a.have_snaps[ks] != 0, a.pool_mark[ks], k, sched[ks], ccur,
-                     stats);
+            adj_step(c, a, st, a.step_snaps[ks], a.have_snaps[ks], a.pool_mark[ks], k, sched[ks], ccur, stats);
             report(c, LBM_PROGRESS_REVERSE_STEP, k, sched[ks]);
             ccur = 1 - ccur;
         }

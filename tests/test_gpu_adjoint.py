@@ -98,6 +98,13 @@ def test_recomputation_does_not_change_the_result():
         if L:
             assert st.store_slots < m and st.recomputed_iterations >= sum(iters), (st.store_slots, m)
         assert J == J0 and k == k0 and np.array_equal(g, g0), (K, L)
+    # Given the iteration counts, the budget split is chosen for the longest
+    # solve, and a solve whose block snapshots do not fit keeps the top level
+    # of its binomial checkpointing from the forward pass.
+    for K, L in ((4, 10), (2, 20), (1, 12)):
+        sim.set_adjoint_memory(budget(L))
+        J, g, k = sim.adjoint(phi0, target, 4, checkpoint_interval=K, iterations=iters)
+        assert J == J0 and k == k0 and np.array_equal(g, g0), ("scheduled", K, L)
 
 
 def test_fp32_state_matches_fp64_state():
