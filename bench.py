@@ -297,6 +297,27 @@ def run_adjoint_falling(hip, steps, checkpoint, state, fd=False, h=1e-6):
     return rec
 
 
+def copy_ceiling():
+    """HBM bandwidths (GB/s) that copies reach on this GPU, by access
+    pattern, from lbm_amd/copy_probe's ceiling mode run as a child process;
+    None if the probe is not built or fails."""
+    import subprocess
+
+    probe = os.path.join(ROOT, "lbm_amd", "copy_probe")
+    if not os.path.exists(probe):
+        return None
+    try:
+        out = subprocess.run([probe, "ceiling"], capture_output=True, text=True, timeout=120).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    rates = {}
+    for line in out.splitlines():
+        name, _, rest = line.partition(":")
+        if rest.strip().endswith("GB/s"):
+            rates[name.strip()] = float(rest.split(",")[1].split()[0])
+    return rates or None
+
+
 def cpu_baseline(hip, wl, workload, n_it_per_step):
     """Single-thread CPU oracle on the same initial state: one non-Poisson step
     part plus two Poisson iterations, extrapolated with the GPU's measured
@@ -462,6 +483,13 @@ def main(argv=None):
                 line["roofline"]["traffic"] = tb
             else:
                 line[f"{prec}_traffic_bytes_per_launch"] = tb
+    # measured on this GPU next to the spec peak: a plain 4 GiB copy, and
+    # copies shaped like the Poisson iteration's 15 + 15 streams
+    ceiling = copy_ceiling()
+    line["hbm_copy_ceiling_gbs"] = ceiling
+    if ceiling:
+        best = max(ceiling.values())
+        line["roofline"]["frac_of_copy_ceiling"] = head["achieved_gbs"] / best
     if not args.no_adjoint:
         line["adjoint"] = run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint)
     if not args.no_cpu_baseline:
