@@ -129,6 +129,7 @@ def run_precision(hip, wl, precision, steps, warmup):
                       check=pf.check_ms, correct=pf.correct_ms),
     )
     out["fields"] = sim.downloadFieldsToCPU()
+    out["mass"] = sim.mass()  # device FP64 sum of phi
     sim.close()
     return out
 
@@ -140,7 +141,7 @@ def field_errors(mixed, fp64):
     import numpy as np
 
     err = {}
-    for k in ("fei", "u", "v", "w", "p"):
+    for k in ("fei", "rho", "u", "v", "w", "p"):
         a, b = mixed[k], fp64[k]
         d = a - b
         err[k] = dict(linf_abs=float(np.abs(d).max()),
@@ -472,6 +473,9 @@ def main(argv=None):
         line["mixed_vs_fp64_error"] = err
         line["spurious_current_max_speed"] = speed
         line["mixed_same_iterations"] = results["mixed"]["iterations"] == results["fp64"]["iterations"]
+        line["mass_sum_phi"] = {p: results[p]["mass"] for p in ("fp64", "mixed")}
+        line["mass_sum_phi"]["mixed_rel_diff"] = abs(results["mixed"]["mass"] - results["fp64"]["mass"]) / abs(
+            results["fp64"]["mass"])
     # HBM bytes per launch of the dominant kernel, from the committed
     # rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a copy-kernel
     # calibration); null when no measurement exists for this workload.

@@ -283,7 +283,8 @@ Ctx* create(const lbm_params_t& P, int precision)
             *a = dalloc_bytes(fb);
         c->p_prev = dalloc(n);
         if (c->mixed) c->scratch1 = dalloc(n);
-        const size_t nparts = std::max<size_t>(2 * static_cast<size_t>(lbm::grid_blocks(c->g.n)),
+        const size_t nparts = std::max<size_t>(2 * static_cast<size_t>(std::max(lbm::grid_blocks(c->g.n),
+                                                                                lbm::stencil_blocks(c->g))),
                                                9 * static_cast<size_t>(lbm::RED_BLOCKS));
         c->partials = dalloc(nparts);
         c->dst = static_cast<DevState*>(dalloc_bytes(sizeof(DevState)));
@@ -368,7 +369,7 @@ void step_fg(Ctx& c)
     c.fcur = nxt;
     c.fg = FG::Post;
     record(c, 1);
-    lbm::f64::source_finalize(s, g, c.ph, c.partials, lbm::grid_blocks(g.n), c.dst);
+    lbm::f64::source_finalize(s, g, c.ph, c.partials, lbm::stencil_blocks(g), c.dst);
     if (!c.mixed) {
         lbm::f64::macro(s, 1, g, c.ph, D<double>(c.f[c.fcur]), D<double>(c.gd[c.fcur]),
                         D<double>(c.ux), D<double>(c.vy), D<double>(c.wz), c.dst, D<double>(c.fei),

@@ -102,6 +102,32 @@ __device__ __forceinline__ int64_t lin(int x, int y, int z, const Grid& g)
     return (static_cast<int64_t>(z) * g.ly + y) * g.lx + x;
 }
 
+// The collision kernels' tile: STENCIL_TX x STENCIL_TY cells (one wave64 per
+// x-row segment) marching through STENCIL_ZCHUNK z layers.  The five stencil
+// fields of each layer are staged in LDS once, with a one-cell halo in x and
+// y, in a ring of three planes (layers z-1, z, z+1 of the cell being
+// collided).
+constexpr int STENCIL_TX = 64, STENCIL_TY = 4, STENCIL_ZCHUNK = 16;
+constexpr int STENCIL_SX = STENCIL_TX + 2, STENCIL_SY = STENCIL_TY + 2;
+constexpr int STENCIL_PLANE = STENCIL_SX * STENCIL_SY;
+
+// Global cell held at slot e of a staged plane of layer gz for the tile at
+// (x0, y0): x mirrored at the walls (as mirror_x), y periodic.
+__device__ __forceinline__ int64_t stencil_source(int e, int x0, int y0, int gz, const Grid& g)
+{
+    const int sx = e % STENCIL_SX, sy = e / STENCIL_SX;
+    int gx = x0 - 1 + sx;
+    gx = gx < 0 ? 1 : (gx >= g.lx ? g.lx - 2 : gx);
+    const int gy = (y0 - 1 + sy + g.ly) % g.ly;
+    return lin(gx, gy, gz, g);
+}
+
+// Slot of neighbour k (k = 0: the cell) of tile cell (tx, ty) within its plane.
+__device__ __forceinline__ int stencil_slot(int tx, int ty, int k)
+{
+    return (tx + 1 + CX[k]) + STENCIL_SX * (ty + 1 + CY[k]);
+}
+
 // Deterministic wave64 butterfly sum.
 __device__ __forceinline__ double wave_sum(double v)
 {

@@ -62,6 +62,12 @@ enum : int { H_STREAMED = 0, H_EQUILIBRIUM = 1, H_GAUGED = 2, H_EXPLICIT = 3 };
 enum : int { FG_FIRST = 0, FG_PULL = 1, FG_LOCAL = 2 };
 
 inline int grid_blocks(int64_t n) { return static_cast<int>((n + BLOCK - 1) / BLOCK); }
+// Blocks (tiles) of the collision kernels; each writes one pair of partials.
+inline int stencil_blocks(const Grid& g)
+{
+    return ((g.lx + STENCIL_TX - 1) / STENCIL_TX) * ((g.ly + STENCIL_TY - 1) / STENCIL_TY) *
+           ((g.lz + STENCIL_ZCHUNK - 1) / STENCIL_ZCHUNK);
+}
 inline int red_blocks(int64_t n)
 {
     const int64_t b = (n + BLOCK - 1) / BLOCK;

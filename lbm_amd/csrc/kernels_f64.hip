@@ -71,8 +71,7 @@ __device__ __forceinline__ double lap(const double (&nb)[Q])
     return (acc - 14.0 * nb[0]) / 5.0;
 }
 
-__device__ __forceinline__ void load_nb(const double* __restrict__ a, const int64_t (&idx)[Q],
-                                        double (&nb)[Q])
+__device__ __forceinline__ void load_nb(const double* a, const int (&idx)[Q], double (&nb)[Q])
 {
 #pragma unroll
     for (int k = 0; k < Q; ++k) nb[k] = a[idx[k]];
@@ -80,7 +79,11 @@ __device__ __forceinline__ void load_nb(const double* __restrict__ a, const int6
 
 // Inamuro::collision, CPU/Inamuro.cpp:134-250, fused with the pull of the
 // streamed f/g (mode FG_PULL) so that one pass reads the stencils, the
-// incoming populations and writes the post-collision populations.
+// incoming populations and writes the post-collision populations.  A block
+// owns a STENCIL_TX x STENCIL_TY tile and marches it through STENCIL_ZCHUNK
+// z layers, staging u, v, w, rho and phi of each layer and its x/y halo in
+// LDS once (a ring of three layers), so the 5 x 15 stencil reads of a cell
+// come from LDS, in the reference's k order, bit for bit.
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_collide(
     Grid g, Phys ph,
@@ -93,101 +96,151 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
     double* __restrict__ partials)
 {
     __shared__ double lds[4];
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    __shared__ double staged[5][3][STENCIL_PLANE];  // u, v, w, rho, phi; ring of 3 layers
+    const int ntx = (g.lx + STENCIL_TX - 1) / STENCIL_TX, nty = (g.ly + STENCIL_TY - 1) / STENCIL_TY;
+    const int b = static_cast<int>(blockIdx.x);
+    const int x0 = (b % ntx) * STENCIL_TX, y0 = ((b / ntx) % nty) * STENCIL_TY;
+    const int zb = (b / (ntx * nty)) * STENCIL_ZCHUNK, ze = min(g.lz, zb + STENCIL_ZCHUNK);
+    // A plane is staged in two halves: its loads go to registers (next[])
+    // before the current layer is collided, and land in LDS after it, so the
+    // loads' latency overlaps the collision arithmetic.
+    constexpr int PER = (STENCIL_PLANE + BLOCK - 1) / BLOCK;
+    double next[5][PER];
+    auto fetch = [&](int gz) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = threadIdx.x + j * BLOCK;
+            if (e < STENCIL_PLANE) {
+                const int64_t i = stencil_source(e, x0, y0, gz, g);
+                next[0][j] = u[i];
+                next[1][j] = v[i];
+                next[2][j] = w[i];
+                next[3][j] = rho[i];
+                next[4][j] = fei[i];
+            }
+        }
+    };
+    auto land = [&](int slot) {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = threadIdx.x + j * BLOCK;
+            if (e < STENCIL_PLANE) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) staged[f][slot][e] = next[f][j];
+            }
+        }
+    };
+    // layer L sits in ring slot (L - zb + 1) mod 3
+    fetch(z_stencil(zb, -1, g));
+    land(0);
+    fetch(zb);
+    land(1);
+    fetch(z_stencil(zb, 1, g));
+    const int tx = threadIdx.x % STENCIL_TX, ty = threadIdx.x / STENCIL_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < g.lx && y < g.ly;
     double div_c = 0.0, p_c = 0.0;
-    if (c < g.n) {
-        int x, y, z;
-        cell_xyz(c, g, x, y, z);
-        int64_t idx[Q];
+    for (int z = zb; z < ze; ++z) {
+        const int sm = (z - zb) % 3, s0 = (z - zb + 1) % 3, sp = (z - zb + 2) % 3;
+        land(sp);
+        __syncthreads();
+        if (z + 1 < ze) fetch(z_stencil(z + 1, 1, g));
+        if (valid) {
+            const int64_t c = lin(x, y, z, g);
+            // flat index into staged[f]: ring slot of the neighbour's layer, then
+            // its place in the plane
+            int idx[Q];
 #pragma unroll
-        for (int k = 0; k < Q; ++k)
-            idx[k] = lin(mirror_x(x, CX[k], g.lx), wrap(y, CY[k], g.ly), z_stencil(z, CZ[k], g), g);
+            for (int k = 0; k < Q; ++k)
+                idx[k] = (CZ[k] == 1 ? sp : (CZ[k] == -1 ? sm : s0)) * STENCIL_PLANE + stencil_slot(tx, ty, k);
 
-        double nb[Q];
-        double u_x, u_y, u_z, v_x, v_y, v_z, w_x, w_y, w_z;
-        double rho_x, rho_y, rho_z, fei_x, fei_y, fei_z;
-        load_nb(u, idx, nb);
-        grad(nb, u_x, u_y, u_z);
-        const double u_lap = lap(nb);
-        const double uc = nb[0];
-        load_nb(v, idx, nb);
-        grad(nb, v_x, v_y, v_z);
-        const double v_lap = lap(nb);
-        const double vc = nb[0];
-        load_nb(w, idx, nb);
-        grad(nb, w_x, w_y, w_z);
-        const double w_lap = lap(nb);
-        const double wc = nb[0];
-        load_nb(rho, idx, nb);
-        grad(nb, rho_x, rho_y, rho_z);
-        const double rhoc = nb[0];
-        load_nb(fei, idx, nb);
-        grad(nb, fei_x, fei_y, fei_z);
-        const double fei_lap = lap(nb);
-        const double feic = nb[0];
+            double nb[Q];
+            double u_x, u_y, u_z, v_x, v_y, v_z, w_x, w_y, w_z;
+            double rho_x, rho_y, rho_z, fei_x, fei_y, fei_z;
+            load_nb(&staged[0][0][0], idx, nb);
+            grad(nb, u_x, u_y, u_z);
+            const double u_lap = lap(nb);
+            const double uc = nb[0];
+            load_nb(&staged[1][0][0], idx, nb);
+            grad(nb, v_x, v_y, v_z);
+            const double v_lap = lap(nb);
+            const double vc = nb[0];
+            load_nb(&staged[2][0][0], idx, nb);
+            grad(nb, w_x, w_y, w_z);
+            const double w_lap = lap(nb);
+            const double wc = nb[0];
+            load_nb(&staged[3][0][0], idx, nb);
+            grad(nb, rho_x, rho_y, rho_z);
+            const double rhoc = nb[0];
+            load_nb(&staged[4][0][0], idx, nb);
+            grad(nb, fei_x, fei_y, fei_z);
+            const double fei_lap = lap(nb);
+            const double feic = nb[0];
 
-        const double mu = (rhoc - ph.rho_G) * ph.tmp1 + ph.mu_G;
-        const double sum_fei = fei_x * fei_x + fei_y * fei_y + fei_z * fei_z;
-        const double sum_rho = rho_x * rho_x + rho_y * rho_y + rho_z * rho_z;
-        const double fei2sum = fei_lap;
-        const double usq = uc * uc + vc * vc + wc * wc;
-        const double p0 = feic * ph.T / (1.0 - ph.b * feic) - ph.a * feic * feic;
+            const double mu = (rhoc - ph.rho_G) * ph.tmp1 + ph.mu_G;
+            const double sum_fei = fei_x * fei_x + fei_y * fei_y + fei_z * fei_z;
+            const double sum_rho = rho_x * rho_x + rho_y * rho_y + rho_z * rho_z;
+            const double fei2sum = fei_lap;
+            const double usq = uc * uc + vc * vc + wc * wc;
+            const double p0 = feic * ph.T / (1.0 - ph.b * feic) - ph.a * feic * feic;
 
-        double fv[Q], gv[Q];
-        if (MODE == FG_PULL) {
-            pull_slip(fin, x, y, z, g, fv);
-            pull_slip(gin, x, y, z, g, gv);
-        } else if (MODE == FG_LOCAL) {
+            double fv[Q], gv[Q];
+            if (MODE == FG_PULL) {
+                pull_slip(fin, x, y, z, g, fv);
+                pull_slip(gin, x, y, z, g, gv);
+            } else if (MODE == FG_LOCAL) {
 #pragma unroll
-            for (int q = 0; q < Q; ++q) {
-                fv[q] = fin[q * g.n + c];
-                gv[q] = gin[q * g.n + c];
+                for (int q = 0; q < Q; ++q) {
+                    fv[q] = fin[q * g.n + c];
+                    gv[q] = gin[q * g.n + c];
+                }
             }
+
+#pragma unroll
+            for (int i = 0; i < Q; ++i) {
+                const double un = UC[i] * uc + VC[i] * vc + WC[i] * wc;
+                const double fei_ei = UC[i] * fei_x + VC[i] * fei_y + WC[i] * fei_z;
+                const double rho_ei = UC[i] * rho_x + VC[i] * rho_y + WC[i] * rho_z;
+                const double ci_sq = UC[i] * UC[i] + VC[i] * VC[i] + WC[i] * WC[i];
+                const double Gfei = 4.5 * (fei_ei * fei_ei) - 1.5 * sum_fei * ci_sq;
+                const double Grho = 4.5 * (rho_ei * rho_ei) - 1.5 * sum_rho * ci_sq;
+                const double fequ = HW[i] * feic +
+                                    FW[i] * (p0 - ph.k_f * feic * fei2sum - ph.k_f * sum_fei / 6.0) +
+                                    3.0 * EW[i] * feic * un +
+                                    EW[i] * ph.k_f * Gfei;
+                const double velPart = 1.0 + 3.0 * un - 1.5 * usq + 4.5 * un * un +
+                    1.5 * (ph.taug - 0.5) * 2.0 *
+                        (UC[i] * UC[i] * u_x + UC[i] * VC[i] * u_y + UC[i] * WC[i] * u_z +
+                         VC[i] * UC[i] * v_x + VC[i] * VC[i] * v_y + VC[i] * WC[i] * v_z +
+                         WC[i] * UC[i] * w_x + WC[i] * VC[i] * w_y + WC[i] * WC[i] * w_z);
+                const double gequ = EW[i] * velPart +
+                                    EW[i] * ph.k_g / rhoc * Grho -
+                                    2.0 / 3.0 * FW[i] * ph.k_g / rhoc * sum_rho;
+                double fo, go;
+                if (MODE == FG_FIRST) {
+                    fo = fequ;
+                    go = gequ;
+                } else {
+                    fo = fv[i] - (fv[i] - fequ) / ph.tauf;
+                    go = gv[i] - (gv[i] - gequ) / ph.taug +
+                         3.0 * EW[i] / rhoc * mu * (UC[i] * u_lap + VC[i] * v_lap + WC[i] * w_lap);
+                }
+                if (CZ[i] != 0 && ph.gz != 0.0) {
+                    // extension: buoyant acceleration gz (1 - rho_G/rho) on the
+                    // predicted velocity
+                    const double az = ph.gz * (1.0 - ph.rho_G / rhoc);
+                    go += 3.0 * EW[i] * WC[i] * az;
+                }
+                fout[i * g.n + c] = fo;
+                gout[i * g.n + c] = go;
+            }
+            ux_o[c] = u_x;
+            vy_o[c] = v_y;
+            wz_o[c] = w_z;
+            div_c += u_x + v_y + w_z;
+            p_c += p[c];
         }
-
-#pragma unroll
-        for (int i = 0; i < Q; ++i) {
-            const double un = UC[i] * uc + VC[i] * vc + WC[i] * wc;
-            const double fei_ei = UC[i] * fei_x + VC[i] * fei_y + WC[i] * fei_z;
-            const double rho_ei = UC[i] * rho_x + VC[i] * rho_y + WC[i] * rho_z;
-            const double ci_sq = UC[i] * UC[i] + VC[i] * VC[i] + WC[i] * WC[i];
-            const double Gfei = 4.5 * (fei_ei * fei_ei) - 1.5 * sum_fei * ci_sq;
-            const double Grho = 4.5 * (rho_ei * rho_ei) - 1.5 * sum_rho * ci_sq;
-            const double fequ = HW[i] * feic +
-                                FW[i] * (p0 - ph.k_f * feic * fei2sum - ph.k_f * sum_fei / 6.0) +
-                                3.0 * EW[i] * feic * un +
-                                EW[i] * ph.k_f * Gfei;
-            const double velPart = 1.0 + 3.0 * un - 1.5 * usq + 4.5 * un * un +
-                1.5 * (ph.taug - 0.5) * 2.0 *
-                    (UC[i] * UC[i] * u_x + UC[i] * VC[i] * u_y + UC[i] * WC[i] * u_z +
-                     VC[i] * UC[i] * v_x + VC[i] * VC[i] * v_y + VC[i] * WC[i] * v_z +
-                     WC[i] * UC[i] * w_x + WC[i] * VC[i] * w_y + WC[i] * WC[i] * w_z);
-            const double gequ = EW[i] * velPart +
-                                EW[i] * ph.k_g / rhoc * Grho -
-                                2.0 / 3.0 * FW[i] * ph.k_g / rhoc * sum_rho;
-            double fo, go;
-            if (MODE == FG_FIRST) {
-                fo = fequ;
-                go = gequ;
-            } else {
-                fo = fv[i] - (fv[i] - fequ) / ph.tauf;
-                go = gv[i] - (gv[i] - gequ) / ph.taug +
-                     3.0 * EW[i] / rhoc * mu * (UC[i] * u_lap + VC[i] * v_lap + WC[i] * w_lap);
-            }
-            if (CZ[i] != 0 && ph.gz != 0.0) {
-                // extension: buoyant acceleration gz (1 - rho_G/rho) on the
-                // predicted velocity
-                const double az = ph.gz * (1.0 - ph.rho_G / rhoc);
-                go += 3.0 * EW[i] * WC[i] * az;
-            }
-            fout[i * g.n + c] = fo;
-            gout[i * g.n + c] = go;
-        }
-        ux_o[c] = u_x;
-        vy_o[c] = v_y;
-        wz_o[c] = w_z;
-        div_c = u_x + v_y + w_z;
-        p_c = p[c];
+        __syncthreads();  // the next layer's staging reuses the oldest slot
     }
     const double sd = block_sum(div_c, lds);
     const double sp = block_sum(p_c, lds);
@@ -652,7 +705,7 @@ void collide(hipStream_t s, int mode, const Grid& g, const Phys& ph, const doubl
              const double* p, const double* fin, const double* gin, double* fout,
              double* gout, double* ux, double* vy, double* wz, double* partials)
 {
-    const int nb = grid_blocks(g.n);
+    const int nb = stencil_blocks(g);
     switch (mode) {
     case FG_FIRST:
         k_collide<FG_FIRST><<<nb, BLOCK, 0, s>>>(g, ph, fei, rho, u, v, w, p, fin, gin, fout, gout,

@@ -110,7 +110,7 @@ struct Stencil {
     float gx, gy, gz, lap, c;
 };
 
-__device__ __forceinline__ Stencil stencil(const float* __restrict__ a, const int (&idx)[Q])
+__device__ __forceinline__ Stencil stencil(const float* a, const int (&idx)[Q])
 {
     float ax = 0.f, ay = 0.f, az = 0.f, s = 0.f;
     float v[Q];
@@ -168,6 +168,9 @@ __device__ __forceinline__ void collide_dir(const PhysF& ph, float feic, float u
     }
 }
 
+// The collision with its stencils (tile and LDS staging as the FP64 kernel:
+// a STENCIL_TX x STENCIL_TY tile marching through STENCIL_ZCHUNK layers, u,
+// v, w, rho and phi of each layer staged once with their halo).
 template <int MODE>
 __global__ __launch_bounds__(BLOCK) void k_collide(
     Grid g, PhysF ph, const float* __restrict__ fei, const float* __restrict__ rho,
@@ -177,69 +180,98 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
     float* __restrict__ vy_o, float* __restrict__ wz_o, double* __restrict__ partials)
 {
     __shared__ double lds[4];
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    __shared__ float staged[5][3][STENCIL_PLANE];  // u, v, w, rho, phi; ring of 3 layers
+    const int ntx = (g.lx + STENCIL_TX - 1) / STENCIL_TX, nty = (g.ly + STENCIL_TY - 1) / STENCIL_TY;
+    const int b = static_cast<int>(blockIdx.x);
+    const int x0 = (b % ntx) * STENCIL_TX, y0 = ((b / ntx) % nty) * STENCIL_TY;
+    const int zb = (b / (ntx * nty)) * STENCIL_ZCHUNK, ze = min(g.lz, zb + STENCIL_ZCHUNK);
+    auto stage = [&](int gz, int slot) {
+        for (int e = threadIdx.x; e < STENCIL_PLANE; e += BLOCK) {
+            const int64_t i = stencil_source(e, x0, y0, gz, g);
+            staged[0][slot][e] = u[i];
+            staged[1][slot][e] = v[i];
+            staged[2][slot][e] = w[i];
+            staged[3][slot][e] = rho[i];
+            staged[4][slot][e] = fei[i];
+        }
+    };
+    // layer L sits in ring slot (L - zb + 1) mod 3; the loads are issued
+    // directly into LDS (holding a plane in registers as the FP64 kernel does
+    // would cost this kernel two thirds of its occupancy)
+    stage(z_stencil(zb, -1, g), 0);
+    stage(zb, 1);
+    const int tx = threadIdx.x % STENCIL_TX, ty = threadIdx.x / STENCIL_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < g.lx && y < g.ly;
     double div_c = 0.0, p_c = 0.0;
-    if (c < g.n) {
-        int x, y, z;
-        cell_xyz(c, g, x, y, z);
-        int idx[Q];
+    for (int z = zb; z < ze; ++z) {
+        const int sm = (z - zb) % 3, s0 = (z - zb + 1) % 3, sp = (z - zb + 2) % 3;
+        stage(z_stencil(z, 1, g), sp);
+        __syncthreads();
+        if (valid) {
+            const int64_t c = lin(x, y, z, g);
+            // flat index into staged[f]: ring slot of the neighbour's layer, then
+            // its place in the plane
+            int idx[Q];
 #pragma unroll
-        for (int k = 0; k < Q; ++k)
-            idx[k] = static_cast<int>(lin(mirror_x(x, CX[k], g.lx), wrap(y, CY[k], g.ly), z_stencil(z, CZ[k], g), g));
-        const Stencil U = stencil(u, idx);
-        const Stencil V = stencil(v, idx);
-        const Stencil W = stencil(w, idx);
-        const Stencil R = stencil(rho, idx);
-        const Stencil F = stencil(fei, idx);
-        const float feic = F.c, rhoc = R.c;
-        const float mu = (rhoc - ph.rho_G) * ph.tmp1 + ph.mu_G;
-        const float sum_fei = F.gx * F.gx + F.gy * F.gy + F.gz * F.gz;
-        const float sum_rho = R.gx * R.gx + R.gy * R.gy + R.gz * R.gz;
-        const float usq = U.c * U.c + V.c * V.c + W.c * W.c;
-        const float p0 = feic * ph.T / (1.0f - ph.b * feic) - ph.a * feic * feic;
-        const float P0p = p0 - ph.k_f * feic * F.lap - ph.k_f * sum_fei * (1.0f / 6.0f);
-        const float inv_rho = 1.0f / rhoc;
-        const float kg_rho = ph.k_g * inv_rho;
-        const float force_scale = inv_rho * mu;
-        float fv[Q], gv[Q];
-        if (MODE == FG_PULL) {
-            pull_slip(fin, x, y, z, g, fv);
-            pull_slip(gin, x, y, z, g, gv);
-        } else if (MODE == FG_LOCAL) {
+            for (int k = 0; k < Q; ++k)
+                idx[k] = (CZ[k] == 1 ? sp : (CZ[k] == -1 ? sm : s0)) * STENCIL_PLANE + stencil_slot(tx, ty, k);
+            const Stencil U = stencil(&staged[0][0][0], idx);
+            const Stencil V = stencil(&staged[1][0][0], idx);
+            const Stencil W = stencil(&staged[2][0][0], idx);
+            const Stencil R = stencil(&staged[3][0][0], idx);
+            const Stencil F = stencil(&staged[4][0][0], idx);
+            const float feic = F.c, rhoc = R.c;
+            const float mu = (rhoc - ph.rho_G) * ph.tmp1 + ph.mu_G;
+            const float sum_fei = F.gx * F.gx + F.gy * F.gy + F.gz * F.gz;
+            const float sum_rho = R.gx * R.gx + R.gy * R.gy + R.gz * R.gz;
+            const float usq = U.c * U.c + V.c * V.c + W.c * W.c;
+            const float p0 = feic * ph.T / (1.0f - ph.b * feic) - ph.a * feic * feic;
+            const float P0p = p0 - ph.k_f * feic * F.lap - ph.k_f * sum_fei * (1.0f / 6.0f);
+            const float inv_rho = 1.0f / rhoc;
+            const float kg_rho = ph.k_g * inv_rho;
+            const float force_scale = inv_rho * mu;
+            float fv[Q], gv[Q];
+            if (MODE == FG_PULL) {
+                pull_slip(fin, x, y, z, g, fv);
+                pull_slip(gin, x, y, z, g, gv);
+            } else if (MODE == FG_LOCAL) {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) {
+                    fv[q] = fin[q * g.n + c];
+                    gv[q] = gin[q * g.n + c];
+                }
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; ++q) fv[q] = gv[q] = 0.f;
+            }
+#define LBM_MX_DIR(I)                                                                           \
+        collide_dir<I, MODE>(ph, feic, U.c, V.c, W.c, U, V, W, R.gx, R.gy, R.gz, F.gx, F.gy, F.gz, \
+                             P0p, usq, sum_fei, sum_rho, kg_rho, force_scale, fv[I], gv[I]);
+            LBM_MX_DIR(0) LBM_MX_DIR(1) LBM_MX_DIR(2) LBM_MX_DIR(3) LBM_MX_DIR(4)
+            LBM_MX_DIR(5) LBM_MX_DIR(6) LBM_MX_DIR(7) LBM_MX_DIR(8) LBM_MX_DIR(9)
+            LBM_MX_DIR(10) LBM_MX_DIR(11) LBM_MX_DIR(12) LBM_MX_DIR(13) LBM_MX_DIR(14)
+#undef LBM_MX_DIR
+            if (ph.gz != 0.f) {
+                // extension: buoyant acceleration gz (1 - rho_G/rho) on the
+                // predicted velocity
+                const float az = ph.gz * (1.0f - ph.rho_G * inv_rho);
+#pragma unroll
+                for (int q = 0; q < Q; ++q)
+                    if (CZ[q] != 0) gv[q] += 3.0f * EWF[q] * static_cast<float>(CZ[q]) * az;
+            }
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
-                fv[q] = fin[q * g.n + c];
-                gv[q] = gin[q * g.n + c];
+                fout[q * g.n + c] = fv[q];
+                gout[q * g.n + c] = gv[q];
             }
-        } else {
-#pragma unroll
-            for (int q = 0; q < Q; ++q) fv[q] = gv[q] = 0.f;
+            ux_o[c] = U.gx;
+            vy_o[c] = V.gy;
+            wz_o[c] = W.gz;
+            div_c += static_cast<double>(U.gx) + static_cast<double>(V.gy) + static_cast<double>(W.gz);
+            p_c += p[c];
         }
-#define LBM_MX_DIR(I)                                                                           \
-    collide_dir<I, MODE>(ph, feic, U.c, V.c, W.c, U, V, W, R.gx, R.gy, R.gz, F.gx, F.gy, F.gz, \
-                         P0p, usq, sum_fei, sum_rho, kg_rho, force_scale, fv[I], gv[I]);
-        LBM_MX_DIR(0) LBM_MX_DIR(1) LBM_MX_DIR(2) LBM_MX_DIR(3) LBM_MX_DIR(4)
-        LBM_MX_DIR(5) LBM_MX_DIR(6) LBM_MX_DIR(7) LBM_MX_DIR(8) LBM_MX_DIR(9)
-        LBM_MX_DIR(10) LBM_MX_DIR(11) LBM_MX_DIR(12) LBM_MX_DIR(13) LBM_MX_DIR(14)
-#undef LBM_MX_DIR
-        if (ph.gz != 0.f) {
-            // extension: buoyant acceleration gz (1 - rho_G/rho) on the
-            // predicted velocity
-            const float az = ph.gz * (1.0f - ph.rho_G * inv_rho);
-#pragma unroll
-            for (int q = 0; q < Q; ++q)
-                if (CZ[q] != 0) gv[q] += 3.0f * EWF[q] * static_cast<float>(CZ[q]) * az;
-        }
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            fout[q * g.n + c] = fv[q];
-            gout[q * g.n + c] = gv[q];
-        }
-        ux_o[c] = U.gx;
-        vy_o[c] = V.gy;
-        wz_o[c] = W.gz;
-        div_c = static_cast<double>(U.gx) + static_cast<double>(V.gy) + static_cast<double>(W.gz);
-        p_c = p[c];
+        __syncthreads();  // the next layer's staging reuses the oldest slot
     }
     const double sd = block_sum(div_c, lds);
     const double sp = block_sum(p_c, lds);
@@ -579,7 +611,7 @@ void collide(hipStream_t s, int mode, const Grid& g, const PhysF& ph, const floa
              const float* fin, const float* gin, float* fout, float* gout, float* ux, float* vy,
              float* wz, double* partials)
 {
-    const int nb = grid_blocks(g.n);
+    const int nb = stencil_blocks(g);
     switch (mode) {
     case FG_FIRST:
         k_collide<FG_FIRST><<<nb, BLOCK, 0, s>>>(g, ph, fei, rho, u, v, w, p, fin, gin, fout, gout, ux, vy, wz, partials);
