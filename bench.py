@@ -158,10 +158,26 @@ def field_errors(mixed, fp64):
 ADJ = dict(L=96, axes=(20.0, 16.0, 16.0), R_target=17.2)
 
 
-def adjoint_fd(hip, steps, checkpoint, directions, h=1e-6):
-    """The adjoint leg's gradient against central differences of the same
-    frozen-count map at full size: `directions` seeded unit-Gaussian
-    directions over phi0 (step h) and the k_g direction (step h)."""
+def _progress(label, t0):
+    """A progress callback for lbm_adjoint that prints at most every 15 s."""
+    last = [time.perf_counter()]
+
+    def report(phase, step, it):
+        now = time.perf_counter()
+        if now - last[0] > 15.0:
+            last[0] = now
+            print(f"{label}: {phase} {step} at iteration {it}, {now - t0:.1f} s", file=sys.stderr, flush=True)
+
+    return report
+
+
+def run_adjoint(hip, steps, checkpoint, fd_directions=0, h=1e-6):
+    """Plain forward of `steps` steps, then lbm_adjoint of the same map with a
+    checkpoint every `checkpoint` steps; cost ratio = reverse sweep (replays
+    and recomputation included) over the plain forward.  With fd_directions,
+    the gradient against central differences of the same frozen-count map:
+    that many seeded unit-Gaussian directions over phi0 (step h) and the k_g
+    direction (step h)."""
     import numpy as np
     from lbm_amd import configs
 
@@ -171,60 +187,60 @@ def adjoint_fd(hip, steps, checkpoint, directions, h=1e-6):
     target = configs.ellipsoid_phi(L, (ADJ["R_target"],) * 3, P)
     sim = hip.InamuroHIP(P)
     sim.setUsePoissonGraph(GRAPHS)
-    J, g, gk = sim.adjoint(phi0, target, steps, checkpoint_interval=checkpoint)
-    its = sim.iteration_history()[-steps:]
-    rng = np.random.default_rng(3)
-    rows = []
-    for i in range(directions):
-        d = rng.standard_normal(phi0.shape)
-        fd = (sim.objective(phi0 + h * d, target, its) - sim.objective(phi0 - h * d, target, its)) / (2 * h)
-        ad = float((g * d).sum())
-        rows.append(dict(fd=fd, adjoint=ad, rel=abs(fd - ad) / abs(fd)))
-        print(f"adjoint fd direction {i}: fd {fd:.10e} adjoint {ad:.10e} rel {rows[-1]['rel']:.2e}",
-              file=sys.stderr, flush=True)
-    sim.close()
-    sp = hip.InamuroHIP(dataclasses.replace(P, k_g=P.k_g + h))
-    sm = hip.InamuroHIP(dataclasses.replace(P, k_g=P.k_g - h))
-    fdk = (sp.objective(phi0, target, its) - sm.objective(phi0, target, its)) / (2 * h)
-    return dict(workload=f"ellipsoid_96, {steps} steps from rest, checkpoint every {checkpoint}", h=h,
-                poisson_iterations_per_step=its, J=J,
-                phi0_directions=rows, max_rel_phi0=max(r["rel"] for r in rows),
-                kg=dict(fd=fdk, adjoint=gk, rel=abs(fdk - gk) / abs(fdk)))
-
-
-def run_adjoint(hip, steps, checkpoint):
-    """Plain forward of `steps` steps, then lbm_adjoint of the same map with a
-    checkpoint every `checkpoint` steps; cost ratio = reverse sweep (replays
-    and recomputation included) over the plain forward."""
-    L = ADJ["L"]
-    P = hip.SolverParams(lx=L, ly=L, lz=L, DD=2.0 * ADJ["R_target"], **PHYSICS)
-    from lbm_amd import configs
-
-    phi0 = configs.ellipsoid_phi(L, ADJ["axes"], P)
-    target = configs.ellipsoid_phi(L, (ADJ["R_target"],) * 3, P)
-    sim = hip.InamuroHIP(P)
-    sim.setUsePoissonGraph(GRAPHS)
     sim.initFromCPU(fei=phi0, rho=sim.initial_rho(phi0), u=0.0, v=0.0, w=0.0, p=0.0)
     sim.synchronize()
     t0 = time.perf_counter()
-    sim.performTimeStepGPU(steps)
-    sim.synchronize()
+    for k0 in range(0, steps, 100):
+        sim.performTimeStepGPU(min(100, steps - k0))
+        sim.synchronize()
+        if steps > 100:
+            print(f"adjoint leg: plain forward {min(steps, k0 + 100)} steps, {time.perf_counter() - t0:.1f} s",
+                  file=sys.stderr, flush=True)
     fwd_ms = (time.perf_counter() - t0) * 1e3
     iters = sim.iteration_history()
-    print(f"adjoint leg: plain forward {fwd_ms:.0f} ms, iterations {iters}", file=sys.stderr, flush=True)
+    print(f"adjoint leg: plain forward {fwd_ms:.0f} ms, iterations {iters[:10]}{'...' if steps > 10 else ''}",
+          file=sys.stderr, flush=True)
+    sim.set_progress(_progress("adjoint leg", time.perf_counter()))
     J, g, gk = sim.adjoint(phi0, target, steps, checkpoint_interval=checkpoint)
     st = sim.adjoint_stats()
-    sim.close()
-    return dict(
+    rec = dict(
         workload=f"ellipsoid_96: 96^3, semi-axes 20/16/16 -> target sphere R=17.2, FP64, {steps} steps from rest, "
                  f"checkpoint every {checkpoint}",
-        steps=steps, checkpoint_interval=checkpoint, poisson_iterations_per_step=iters,
+        steps=steps, checkpoint_interval=checkpoint,
+        poisson_iterations_per_step=iters if steps <= 20 else None,
+        poisson_iterations_total=int(sum(iters)),
         forward_ms=fwd_ms, adjoint_forward_ms=st.forward_ms, reverse_ms=st.reverse_ms,
         adjoint_over_forward=st.reverse_ms / fwd_ms,
         total_over_forward=(st.forward_ms + st.reverse_ms) / fwd_ms,
         recomputed_iterations=int(st.recomputed_iterations), adjoint_iterations=int(st.adjoint_iterations),
         store_slots=int(st.store_slots), J=J, dJ_dkg=gk, grad_phi0_linf=float(abs(g).max()),
     )
+    if fd_directions:
+        its = sim.iteration_history()[-steps:]
+        rng = np.random.default_rng(3)
+        rows = []
+        for i in range(fd_directions):
+            d = rng.standard_normal(phi0.shape)
+            jp = sim.objective(phi0 + h * d, target, its)
+            print(f"adjoint fd direction {i}: J(+h) {jp:.12e}", file=sys.stderr, flush=True)
+            jm = sim.objective(phi0 - h * d, target, its)
+            fd = (jp - jm) / (2 * h)
+            ad = float((g * d).sum())
+            rows.append(dict(fd=fd, adjoint=ad, rel=abs(fd - ad) / abs(fd)))
+            print(f"adjoint fd direction {i}: fd {fd:.10e} adjoint {ad:.10e} rel {rows[-1]['rel']:.2e}",
+                  file=sys.stderr, flush=True)
+        jk = []
+        for kg in (P.k_g + h, P.k_g - h):
+            x = hip.InamuroHIP(dataclasses.replace(P, k_g=kg))
+            x.setUsePoissonGraph(GRAPHS)
+            jk.append(x.objective(phi0, target, its))
+            print(f"adjoint fd k_g = {kg:.6e}: J {jk[-1]:.12e}", file=sys.stderr, flush=True)
+            x.close()
+        fdk = (jk[0] - jk[1]) / (2 * h)
+        rec["finite_differences"] = dict(h=h, phi0_directions=rows, max_rel_phi0=max(r["rel"] for r in rows),
+                                         kg=dict(fd=fdk, adjoint=gk, rel=abs(fdk - gk) / abs(fdk)))
+    sim.close()
+    return rec
 
 
 def run_adjoint_falling(hip, steps, checkpoint, state, fd=False, h=1e-6):
@@ -403,10 +419,7 @@ def main(argv=None):
         print(json.dumps(rec), flush=True)
         return 0
     if args.adjoint_only:
-        rec = run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint)
-        if args.adjoint_fd:
-            rec["finite_differences"] = adjoint_fd(hip, args.adjoint_steps, args.adjoint_checkpoint,
-                                                   args.adjoint_fd)
+        rec = run_adjoint(hip, args.adjoint_steps, args.adjoint_checkpoint, fd_directions=args.adjoint_fd)
         print(json.dumps(rec), flush=True)
         return 0
     wl = WORKLOADS[args.workload]
