@@ -4,7 +4,10 @@
 
 The 64^3 config-0 droplet (tanh R=16, density ratio 1000) for `steps` steps
 (default 20), the oracle's per-cell loops on `threads` workers (default 1;
-its global sums stay sequential, so the output does not depend on it).  The compiled reference takes over an hour for the first three
+its global sums stay sequential, so the output does not depend on it).  A
+step whose solve fails (the model going unstable) ends the run; the fixture
+then holds the steps before it and records the failure.  The compiled
+reference takes over an hour for the first three
 steps of this case, so only those three are reference-generated
 (config0_64.json, where the oracle is asserted bit-identical to it); this
 fixture continues the same trajectory with the oracle, whose first three
@@ -43,16 +46,26 @@ def main(argv):
     assert init_sha == base["init_sha256"]
     iters, step_sha = [], []
     t0 = time.time()
+    diverged = None
     for s in range(steps):
-        iters += o.step(1)
-        step_sha.append({k: sha(v) for k, v in o.fields().items()})
+        try:
+            iters += o.step(1)
+        except po.OracleError as e:
+            # the step's solve failed (the model went unstable): keep the
+            # steps before it
+            diverged = dict(step=s + 1, error=str(e))
+            print(f"step {s + 1}: {e}", flush=True)
+            break
+        last = o.fields()
+        step_sha.append({k: sha(v) for k, v in last.items()})
         if s < base["steps"]:
             assert iters[s] == base["iterations"][s] and step_sha[s] == base["step_sha256"][s], s
         print(f"step {s + 1}: {iters[-1]} iterations, {time.time() - t0:.0f} s", flush=True)
-    final = o.fields()
+    steps = len(iters)
+    final = last  # the fields after the last step that completed
     n = P.lx * P.ly * P.lz
     idx = np.sort(np.random.default_rng(20261203).choice(n, size=4096, replace=False))
-    rec = dict(case=f"config0_64_oracle{steps}", generated_by="oracle",
+    rec = dict(case=f"config0_64_oracle{len(step_sha)}", generated_by="oracle", diverged=diverged,
                params=base["params"], init=base["init"], steps=steps, iterations=iters,
                oracle_seconds=time.time() - t0, fields={}, init_sha256=init_sha, step_sha256=step_sha,
                sample_index_seed=20261203)
@@ -62,7 +75,7 @@ def main(argv):
         rec["fields"][k] = dict(sha256=sha(final[k]), sum=float(a.sum()), sum_abs=float(np.abs(a).sum()),
                                 max_abs=float(np.abs(a).max()), min=float(a.min()), max=float(a.max()))
         samples[k] = a[idx].copy()
-    name = f"config0_64_oracle{steps}"
+    name = f"config0_64_oracle{len(step_sha)}"
     json.dump(rec, open(os.path.join(OUT, name + ".json"), "w"), indent=1)
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **samples)
     print(f"wrote {name}: iterations {iters}", flush=True)
