@@ -214,6 +214,8 @@ def run_adjoint(hip, steps, checkpoint, fd_directions=0, h=1e-6):
         total_over_forward=(st.forward_ms + st.reverse_ms) / fwd_ms,
         recomputed_iterations=int(st.recomputed_iterations), adjoint_iterations=int(st.adjoint_iterations),
         store_slots=int(st.store_slots), J=J, dJ_dkg=gk, grad_phi0_linf=float(abs(g).max()),
+        reverse_solve_final_relative_change=st.final_relative_change,
+        reverse_solve_iterations_to_tolerance=int(st.iterations_to_tolerance),
     )
     if fd_directions:
         its = sim.iteration_history()[-steps:]
@@ -301,6 +303,8 @@ def run_adjoint_falling(hip, steps, checkpoint, state, fd=False, h=1e-6):
         total_over_forward=(st.forward_ms + st.reverse_ms) / fwd_ms,
         recomputed_iterations=int(st.recomputed_iterations), adjoint_iterations=int(st.adjoint_iterations),
         store_slots=int(st.store_slots), J=J, dJ_dkg=gk, grad_phi0_linf=float(abs(g).max()),
+        reverse_solve_final_relative_change=st.final_relative_change,
+        reverse_solve_iterations_to_tolerance=int(st.iterations_to_tolerance),
     )
     if fd:
         vals = []

@@ -113,6 +113,14 @@ typedef struct lbm_adjoint_stats_s {
     uint64_t adjoint_iterations;     /* adjoint Poisson iterations */
     double forward_ms;               /* forward pass with checkpoint saves */
     double reverse_ms;               /* reverse sweep, replays and recomputation included */
+    /* Convergence of the reverse Poisson solves, as the forward check measures
+     * its own (sum|p - p_prev| / sum|p| per check interval), on the
+     * accumulated source adjoint lambda_S: the largest over the steps whose
+     * solve spans more than one check interval of the relative change over
+     * the last reversed interval, and of the reverse iterations until it first
+     * fell below the tolerance (-1: some step's never did). */
+    double final_relative_change;
+    int64_t iterations_to_tolerance;
 } lbm_adjoint_stats_t;
 
 /* InamuroCUDA(const Inamuro&) (GPU/InamuroCUDA.hpp:46): allocates device state

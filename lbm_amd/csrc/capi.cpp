@@ -724,6 +724,8 @@ struct AdjWork {
     double *lfei = nullptr, *lrho = nullptr, *lu = nullptr, *lv = nullptr, *lw = nullptr;
     double *lp = nullptr, *lps = nullptr, *lpf = nullptr, *lS = nullptr, *ltau = nullptr;
     double *lmac = nullptr, *pc = nullptr, *parts = nullptr, *target = nullptr;
+    double* lS_prev = nullptr;  // lS at the previous reversed check (convergence record)
+    double* conv = nullptr;     // per step: final relative change, reverse iterations to tolerance
     void* Lb[2] = {nullptr, nullptr};  // adjoint Poisson carriers, FP64 or FP32
     double* cf[2] = {nullptr, nullptr};
     double* cg[2] = {nullptr, nullptr};
@@ -740,7 +742,8 @@ struct AdjWork {
         for (auto& s : seg) F(s.buf);
         F(final_state.buf);
         F(pfinal); F(slots); F(pstore); F(shifts); F(snap_pool); F(scratch_snap);
-        for (double** p : {&lfei, &lrho, &lu, &lv, &lw, &lp, &lps, &lpf, &lS, &ltau, &lmac, &pc, &parts, &target})
+        for (double** p : {&lfei, &lrho, &lu, &lv, &lw, &lp, &lps, &lpf, &lS, &ltau, &lmac, &pc, &parts, &target,
+                           &lS_prev, &conv})
             F(*p);
         for (int i = 0; i < 2; ++i) { F(Lb[i]); F(cf[i]); F(cg[i]); }
         F(as);
@@ -987,6 +990,9 @@ void adj_block(Ctx& c, AdjWork& a, const StepRev& r, uint64_t j, double* start)
                           shift, D<double>(c.tau), a.lS, a.ltau, k == 1 ? nullptr : a.Lb[1 - a.lcur],
                           k == 1 ? a.lps : nullptr, a.parts);
         if (k > 1) a.lcur = 1 - a.lcur;
+        if ((k - 1) % ci == 0)  // a check interval of the solve reversed
+            lbm::adj::converge(s, g, a.lS, a.lS_prev, nit - k + 1, c.tol, k + ci - 1 == nit ? 1 : 0, a.parts,
+                               a.as);
     }
     HIPCHK(hipGetLastError());
 }
@@ -1073,6 +1079,7 @@ void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*
     step_fg(c);  // fei', rho', u', tau_h, S
     const uint64_t nblocks = (nit + L - 1) / L;
     HIPCHK(hipMemsetAsync(a.lS, 0, n * sizeof(double), s));
+    HIPCHK(hipMemsetAsync(a.lS_prev, 0, n * sizeof(double), s));
     HIPCHK(hipMemsetAsync(a.ltau, 0, n * sizeof(double), s));
     a.lcur = 0;
     const StepRev r{p_start, nit, step, &stats};
@@ -1083,6 +1090,7 @@ void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*
     } else {
         adj_reverse_range(c, a, r, 0, nblocks, nullptr, pool_free);
     }
+    lbm::adj::converge_record(s, a.as, a.conv, step);
     stats.adjoint_iterations += nit;
     lbm::adj::source(s, g, c.ph, D<double>(c.fei), D<double>(c.rho), a.ltau, a.lS, a.lfei, a.as, a.lrho,
                      a.lps, a.lmac, a.parts);
@@ -1125,8 +1133,10 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     a.seg.resize(static_cast<size_t>(std::min(K, T)));
     for (auto& st : a.seg) alloc_state(c, st);
     a.pfinal = dalloc(static_cast<size_t>(T) * n);
-    for (double** p : {&a.lfei, &a.lrho, &a.lu, &a.lv, &a.lw, &a.lp, &a.lps, &a.lpf, &a.lS, &a.ltau, &a.lmac, &a.target})
+    for (double** p : {&a.lfei, &a.lrho, &a.lu, &a.lv, &a.lw, &a.lp, &a.lps, &a.lpf, &a.lS, &a.ltau, &a.lmac, &a.target,
+                       &a.lS_prev})
         *p = dalloc(n);
+    a.conv = dalloc(2 * static_cast<size_t>(T));
     for (int i = 0; i < 2; ++i) {
         a.Lb[i] = dalloc_bytes(lbm::Q * n * (c.adj_state_f32 ? sizeof(float) : sizeof(double)));
         a.cf[i] = dalloc(lbm::Q * n);
@@ -1282,12 +1292,24 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
         }
     }
     HIPCHK(hipMemcpyAsync(a.has, a.as, sizeof(lbm::AdjState), hipMemcpyDeviceToHost, s));
-    std::vector<double> lf(n), lr(n);
+    std::vector<double> lf(n), lr(n), conv(2 * static_cast<size_t>(T));
     HIPCHK(hipMemcpyAsync(lf.data(), a.lfei, n * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(lr.data(), a.lrho, n * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(conv.data(), a.conv, conv.size() * sizeof(double), hipMemcpyDeviceToHost, s));
     sync(c);
     auto t2 = std::chrono::steady_clock::now();
     stats.reverse_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    // the reverse solves' convergence: the worst step (a solve of one check
+    // interval has no earlier interval to compare with)
+    for (int k = 0; k < T; ++k) {
+        if (sched[static_cast<size_t>(k)] <= ci) continue;
+        const double change = conv[2 * static_cast<size_t>(k)], reached = conv[2 * static_cast<size_t>(k) + 1];
+        stats.final_relative_change = std::max(stats.final_relative_change, change);
+        if (reached < 0.0 || stats.iterations_to_tolerance < 0)
+            stats.iterations_to_tolerance = -1;
+        else
+            stats.iterations_to_tolerance = std::max(stats.iterations_to_tolerance, static_cast<int64_t>(reached));
+    }
     // phi0 enters as fei and through rho0.
     const double scale = (P.rho_L - P.rho_G) / (P.fei_max - P.fei_min);
     if (grad_phi0)

@@ -121,6 +121,8 @@ struct AdjState {
     double lam_target;  // adjoint of the gauge target mean(p_start), this step
     double lS_mean;     // mean of the source adjoint (projection S - mean)
     double weight;      // centroid objective: sum of the weights phi - fei_G
+    double conv_change;  // relative change of lS over the last reversed check interval
+    double conv_reached; // reverse iterations until it first fell below the tolerance (-1: not yet)
 };
 // Objective of the adjoint: sum (phi_T - target)^2, or the centroid height
 // sum z w / sum w with w = phi_T - fei_G and z the layer index.
@@ -150,6 +152,14 @@ void collide(hipStream_t s, int first, const Grid& g, const Phys& ph, const doub
              const double* cg, const double* lmac, const double* lu, const double* lv,
              const double* lw, const double* lS, AdjState* as, double* cf_o, double* cg_o,
              double* pc, double* parts);
+// Convergence of the reverse solve: after the adjoints of a check interval
+// (down to iteration k of nit), the relative change sum|lS - lS_prev| /
+// sum|lS| of the accumulated source adjoint since the previous one; lS_prev
+// takes lS.  The first call of a step passes first = 1 (resets the record).
+void converge(hipStream_t s, const Grid& g, const double* lS, double* lS_prev, uint64_t done, double tol,
+              int first, double* parts, AdjState* as);
+// Appends (conv_change, conv_reached) of the step just reversed to out[2 * step].
+void converge_record(hipStream_t s, const AdjState* as, double* out, int step);
 void stencil(hipStream_t s, const Grid& g, const double* pc, double* lfei, double* lrho, double* lu,
              double* lv, double* lw);
 constexpr int PC_SLOTS = 24;  // per-cell adjoint slots of collide -> stencil

@@ -409,6 +409,52 @@ __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
     }
 }
 
+// ---- convergence of the reverse solve -------------------------------------------
+
+__global__ __launch_bounds__(BLOCK) void k_adj_change(Grid g, const double* __restrict__ lS,
+                                                      double* __restrict__ lS_prev,
+                                                      double* __restrict__ parts)
+{
+    __shared__ double lds[4];
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    double d = 0.0, a = 0.0;
+    if (c < g.n) {
+        const double v = lS[c];
+        d = fabs(v - lS_prev[c]);
+        a = fabs(v);
+        lS_prev[c] = v;
+    }
+    const double sd = block_sum(d, lds);
+    const double sa = block_sum(a, lds);
+    if (threadIdx.x == 0) {
+        parts[blockIdx.x] = sd;
+        parts[gridDim.x + blockIdx.x] = sa;
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_adj_change_finalize(const double* __restrict__ parts, int nparts,
+                                                               double done, double tol, int first,
+                                                               AdjState* as)
+{
+    __shared__ double lds[4];
+    const double sd = reduce_partials(parts, nparts, lds);
+    const double sa = reduce_partials(parts + nparts, nparts, lds);
+    if (threadIdx.x == 0) {
+        const double rel = sa > 0.0 ? sd / sa : 0.0;
+        if (first) as->conv_reached = -1.0;
+        as->conv_change = rel;
+        if (rel < tol && as->conv_reached < 0.0) as->conv_reached = done;
+    }
+}
+
+__global__ void k_adj_change_record(const AdjState* __restrict__ as, double* __restrict__ out, int step)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        out[2 * step] = as->conv_change;
+        out[2 * step + 1] = as->conv_reached;
+    }
+}
+
 // ---- entry of the solve and getMacro ------------------------------------------
 
 // After all iterations: lps += lambda_target / N (gauge target = mean p_start),
@@ -780,6 +826,19 @@ void collide(hipStream_t s, int first, const Grid& g, const Phys& ph, const doub
         k_adj_collide<0><<<nb, BLOCK, 0, s>>>(g, ph, fei, rho, u, v, w, cf, cg, lmac, lu, lv, lw, lS,
                                               as, cf_o, cg_o, pc, parts);
     k_adj_kg<<<1, BLOCK, 0, s>>>(parts, nb, as);
+}
+
+void converge(hipStream_t s, const Grid& g, const double* lS, double* lS_prev, uint64_t done, double tol,
+              int first, double* parts, AdjState* as)
+{
+    const int nb = grid_blocks(g.n);
+    k_adj_change<<<nb, BLOCK, 0, s>>>(g, lS, lS_prev, parts);
+    k_adj_change_finalize<<<1, BLOCK, 0, s>>>(parts, nb, static_cast<double>(done), tol, first, as);
+}
+
+void converge_record(hipStream_t s, const AdjState* as, double* out, int step)
+{
+    k_adj_change_record<<<1, 64, 0, s>>>(as, out, step);
 }
 
 void stencil(hipStream_t s, const Grid& g, const double* pc, double* lfei, double* lrho, double* lu,

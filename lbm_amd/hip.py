@@ -100,6 +100,7 @@ class AdjointStats(ctypes.Structure):
         ("store_slots", ctypes.c_int64), ("forward_iterations", ctypes.c_uint64),
         ("recomputed_iterations", ctypes.c_uint64), ("adjoint_iterations", ctypes.c_uint64),
         ("forward_ms", ctypes.c_double), ("reverse_ms", ctypes.c_double),
+        ("final_relative_change", ctypes.c_double), ("iterations_to_tolerance", ctypes.c_int64),
     ]
 
 

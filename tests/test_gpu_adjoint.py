@@ -66,6 +66,10 @@ def test_adjoint_matches_oracle(rho_L, ci, steps, cx, lx):
     st = sim.adjoint_stats()
     assert st.steps == steps and st.forward_iterations == sum(iters)
     assert st.adjoint_iterations == sum(iters)
+    # the reverse solves' convergence record is a relative change and an
+    # iteration count within the solves
+    assert 0.0 <= st.final_relative_change <= 1.0
+    assert -1 <= st.iterations_to_tolerance <= max(iters)
 
 
 def test_recomputation_does_not_change_the_result():
