@@ -379,6 +379,9 @@ def cpu_baseline(hip, wl, workload, n_it_per_step):
                 f"({t_np:.2f} s) and {k} Poisson iterations ({t_it:.3f} s each) timed on one host core, "
                 f"extrapolated to {n_it_per_step:.0f} iterations per step (the GPU's measured mean)"),
         host_cpu=platform.processor() or platform.machine(), nproc=os.cpu_count(),
+        # measured where the reference compiles (one core, same inputs): the
+        # oracle takes 0.58-0.60x the compiled reference CPU solver's time
+        reference_estimate=0.59 * cells / t_step / 1e6,
     )
 
 
