@@ -1,7 +1,7 @@
 """Mixed precision (FP32 deviations + FP64 pressure and sums) against the
 FP64 oracle.
 
-Bar (north star): relative L-infinity error of phi, u and p <= 1e-5 against
+Bar: relative L-infinity error of phi, u and p <= 1e-5 against
 the FP64 reference; the same Poisson iteration counts on these cases.
 """
 import numpy as np

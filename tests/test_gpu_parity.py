@@ -1,6 +1,6 @@
 """Parity of the HIP FP64 path with the oracle and the reference fixtures.
 
-Bar (north star): <= 1e-12 relative L-infinity on every field and identical
+Bar: <= 1e-12 relative L-infinity on every field and identical
 Poisson iteration counts per step.  Relative L-infinity is
 max|gpu - ref| / max|ref| over the grid (or over the fixture's sampled cells,
 scaled by the full field's max |ref|).
