@@ -62,7 +62,7 @@ CASES = {
                                     steps=3, samples=1024),
     # Config 0 at full size, first three steps (100, then tens of thousands of
     # Poisson iterations per step).
-    "config0_64": dict(params=dict(lx=64, ly=64, lz=64, DD=32.0, **C0),
+    "static_droplet_64": dict(params=dict(lx=64, ly=64, lz=64, DD=32.0, **C0),
                        init=dict(kind="tanh", center=(32.0, 32.0, 32.0), width=2.0),
                        steps=3, samples=4096),
 }
@@ -182,7 +182,7 @@ def main(argv):
         i = args.index("--reuse")
         reuse = args[i + 1]
         del args[i:i + 2]
-    names = args or [k for k in CASES if k != "config0_64"]
+    names = args or [k for k in CASES if k != "static_droplet_64"]
     for name in names:
         run_case(name, CASES[name], reuse)
 

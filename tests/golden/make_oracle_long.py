@@ -1,17 +1,17 @@
-"""Writes the long config-0 fixture from the oracle alone:
+"""Writes the long 64^3 static-droplet fixture from the oracle alone:
 
     python tests/golden/make_oracle_long.py [steps] [threads]
 
-The 64^3 config-0 droplet (tanh R=16, density ratio 1000) for `steps` steps
+The 64^3 static droplet (tanh R=16, density ratio 1000) for `steps` steps
 (default 20), the oracle's per-cell loops on `threads` workers (default 1;
 its global sums stay sequential, so the output does not depend on it).  A
 step whose solve fails (the model going unstable) ends the run; the fixture
 then holds the steps before it and records the failure.  The compiled
 reference takes over an hour for the first three
 steps of this case, so only those three are reference-generated
-(config0_64.json, where the oracle is asserted bit-identical to it); this
+(static_droplet_64.json, where the oracle is asserted bit-identical to it); this
 fixture continues the same trajectory with the oracle, whose first three
-steps it re-checks against config0_64.json.  Same schema as make_golden.py's
+steps it re-checks against static_droplet_64.json.  Same schema as make_golden.py's
 fixtures, with "generated_by": "oracle".
 """
 import hashlib
@@ -37,7 +37,7 @@ def sha(a):
 def main(argv):
     steps = int(argv[1]) if len(argv) > 1 else 20
     threads = int(argv[2]) if len(argv) > 2 else 1
-    base = json.load(open(os.path.join(OUT, "config0_64.json")))
+    base = json.load(open(os.path.join(OUT, "static_droplet_64.json")))
     P = po.Params(**base["params"])
     o = po.Oracle(P, threads=threads)
     cx, cy, cz = base["init"]["center"]
@@ -65,7 +65,7 @@ def main(argv):
     final = last  # the fields after the last step that completed
     n = P.lx * P.ly * P.lz
     idx = np.sort(np.random.default_rng(20261203).choice(n, size=4096, replace=False))
-    rec = dict(case=f"config0_64_oracle{len(step_sha)}", generated_by="oracle", diverged=diverged,
+    rec = dict(case=f"static_droplet_64_oracle{len(step_sha)}", generated_by="oracle", diverged=diverged,
                params=base["params"], init=base["init"], steps=steps, iterations=iters,
                oracle_seconds=time.time() - t0, fields={}, init_sha256=init_sha, step_sha256=step_sha,
                sample_index_seed=20261203)
@@ -75,7 +75,7 @@ def main(argv):
         rec["fields"][k] = dict(sha256=sha(final[k]), sum=float(a.sum()), sum_abs=float(np.abs(a).sum()),
                                 max_abs=float(np.abs(a).max()), min=float(a.min()), max=float(a.max()))
         samples[k] = a[idx].copy()
-    name = f"config0_64_oracle{len(step_sha)}"
+    name = f"static_droplet_64_oracle{len(step_sha)}"
     json.dump(rec, open(os.path.join(OUT, name + ".json"), "w"), indent=1)
     np.savez_compressed(os.path.join(OUT, name + ".npz"), **samples)
     print(f"wrote {name}: iterations {iters}", flush=True)

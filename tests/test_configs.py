@@ -67,7 +67,7 @@ def test_perturbation_rms_is_sigma_and_mean_free():
 
 
 def test_coefficients_match_the_committed_fixture():
-    rec = json.load(open(os.path.join(GOLDEN, "config2_sh_coefficients.json")))
+    rec = json.load(open(os.path.join(GOLDEN, "head_on_sh_coefficients.json")))
     assert (rec["seed"], rec["lmin"], rec["lmax"]) == (configs.SH_SEED, configs.SH_LMIN, configs.SH_LMAX)
     assert [tuple(m) for m in rec["modes"]] == configs.sh_modes()
     for a, hexes in zip(configs.sh_coefficients(2), rec["coefficients"]):

@@ -21,7 +21,7 @@ GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 # (case, steps replayed here) -- sized so the CPU suite stays short.
 CASES = [("smoke_16x32x32", 5), ("droplet_20x16x28_offset", 2), ("droplet_24_ratio1000", 2),
-         ("config0_64", 1)]
+         ("static_droplet_64", 1)]
 
 
 def sha(a):
