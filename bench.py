@@ -17,6 +17,13 @@ iteration): algorithmic bytes per launch (cells x bytes-per-cell of the
 Poisson iteration byte model) over its average launch time measured with
 HIP events on the library's stream.  `cpu_baseline` times the CPU oracle
 (single thread) on a bounded sample of the same 256^3 state.
+`hbm_copy_ceiling_gbs` is what plain copies reach on the same GPU.  The
+`adjoint` record is the discrete adjoint on the 96^3 ellipsoid over 8 steps.
+
+Other legs: --workload head_on_collision | falling_droplet | static_droplet_64
+(...); --adjoint-only [--adjoint-steps T --adjoint-checkpoint K --adjoint-fd N];
+--adjoint-falling T (the falling droplet's centroid-height adjoint at full
+size); --traffic-probe (one step per precision, for counter runs).
 """
 from __future__ import annotations
 
