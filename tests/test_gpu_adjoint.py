@@ -131,6 +131,37 @@ def test_fp32_state_matches_fp64_state():
     assert abs(k32 - k64) <= 1e-6 * abs(k64), (k32, k64)
 
 
+def test_progress_reports_and_setting_validation():
+    """lbm_set_progress: one report per forward-pass step and per reversed
+    step, in order, and block reports inside the reverse sweep; the objective
+    and state-precision setters reject unknown values."""
+    announce("adjoint progress reports")
+    P = params()
+    phi0, target, _ = case(P)
+    sim = hip.InamuroHIP(P)
+    sim.setPoisson(check_interval=10)
+    calls = []
+    sim.set_progress(lambda phase, step, it: calls.append((phase, step, it)))
+    sim.adjoint(phi0, target, 3, checkpoint_interval=2)
+    iters = sim.iteration_history()[-3:]
+    fwd = [c for c in calls if c[0] == "forward step"]
+    rev = [c for c in calls if c[0] == "reverse step"]
+    assert fwd == [("forward step", k, iters[k]) for k in range(3)]
+    assert rev == [("reverse step", k, iters[k]) for k in (2, 1, 0)]
+    assert any(c[0] == "reverse block" for c in calls)
+    sim.set_progress(None)
+    n = len(calls)
+    sim.adjoint(phi0, target, 1, checkpoint_interval=1)
+    assert len(calls) == n
+    for bad in (lambda: load().lbm_set_adjoint_objective(sim._h, 7),
+                lambda: load().lbm_set_adjoint_state_precision(sim._h, 3)):
+        assert bad() == hip.LBM_ERR_ARG
+
+
+def load():
+    return hip.load_library()
+
+
 def test_context_state_after_adjoint_is_the_forward_end():
     P = params()
     phi0, target, _ = case(P)
