@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 1
+#define LBM_ABI_VERSION 2
 
 /* Status codes. */
 #define LBM_OK            0
@@ -61,8 +61,12 @@ typedef struct lbm_params_s {
     double DD;
 } lbm_params_t;
 
-/* Mirrors InamuroCUDA::PoissonStepDiagnostics (GPU/InamuroCUDA.hpp:19-43) plus
- * the CPU solver's PressureSolveDiagnostics counters (CPU/Inamuro.hpp:76-83). */
+/* Mirrors InamuroCUDA::PoissonStepDiagnostics (GPU/InamuroCUDA.hpp:19-43),
+ * every field with the reference's meaning, plus the CPU solver's
+ * PressureSolveDiagnostics counters (CPU/Inamuro.hpp:76-83).  As in the
+ * reference (GPU/InamuroCUDA.cu:1829-1844) the fixed_point_* values are NaN
+ * when the kinetic residual was not evaluated; the reference's bools are
+ * int32 here. */
 typedef struct lbm_poisson_diag_s {
     uint64_t iterations;
     uint64_t prev_iterations;
@@ -79,6 +83,8 @@ typedef struct lbm_poisson_diag_s {
     double fixed_point_p_relative;
     double fixed_point_relative;
     double source_mean;            /* -mean(div)/3, GPU/InamuroCUDA.cu:1891          */
+    double source_abs_mean;        /* mean|div|/3 before the projection (:1893)        */
+    double projected_source_mean;  /* -mean(div)/3 after the projection (:1920)        */
     double divergence_mean;        /* mean(u_x+v_y+w_z) removed from u_x              */
     double pressure_gauge_target;
     double pressure_gauge_max_shift;
@@ -87,8 +93,35 @@ typedef struct lbm_poisson_diag_s {
     int32_t fixed_point_converged;
     int32_t dual_residual_enabled;
     int32_t converged;
-    int32_t reserved;
+    int32_t fallback_count;        /* 1: stopped at the iteration limit (:2189, :2337) */
 } lbm_poisson_diag_t;
+
+/* One convergence check of the Poisson loop: the values of one row of the
+ * reference's Poisson diagnostics CSV (writePoissonDiagnostic,
+ * GPU/InamuroCUDA.cu:2634-2682, called from check_residual at :2136-2147).
+ * fixed_point_* are NaN where the kinetic residual is not evaluated. */
+typedef struct lbm_poisson_check_s {
+    int64_t step;                  /* 1-based step (perf.time_step_count + 1)          */
+    uint64_t iteration;            /* iterations of this solve so far                  */
+    double pressure_l1_delta;
+    double pressure_l1_norm;
+    double relative_error;
+    double fixed_point_h_l1;
+    double fixed_point_h_scale;
+    double fixed_point_h_relative;
+    double fixed_point_p_l1;
+    double fixed_point_p_scale;
+    double fixed_point_p_relative;
+    double fixed_point_relative;
+    int32_t fixed_point_evaluated;
+    int32_t pressure_converged;
+    int32_t fixed_point_converged;
+    int32_t dual_residual_enabled;
+    int32_t converged;
+    int32_t pad;
+} lbm_poisson_check_t;
+
+typedef void (*lbm_check_fn)(void* user, const lbm_poisson_check_t* check);
 
 /* Event-timed totals (reference: PerformanceMetrics, GPU/InamuroCUDA.hpp:115-132). */
 typedef struct lbm_perf_s {
@@ -174,7 +207,10 @@ int lbm_upload_dists(void* ctx, const double* f, const double* g, const double* 
  * iteration_limit = 0 is the unbounded contract; a nonzero limit must be a
  * multiple of check_interval.  dual_residual: 0 off, 1 exit requires both the
  * relative pressure change and the kinetic fixed-point residual below the
- * tolerance, 2 evaluate the fixed-point residual without changing the exit. */
+ * tolerance, 2 evaluate the fixed-point residual without changing the exit.
+ * All arguments are checked before anything changes: a rejected call leaves
+ * the settings as they were.  It is rejected too when a stored iteration
+ * schedule holds counts that are not multiples of the new check_interval. */
 int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance,
                     uint64_t iteration_limit, int dual_residual);
 
@@ -200,7 +236,10 @@ int lbm_set_gravity(void* ctx, double gz);
  * after this call runs exactly iterations[i] iterations, gauging at every
  * check, whatever the residual; later steps converge as usual.  This makes the
  * step a smooth map of its inputs (the stopping test is not), which is what
- * finite-difference checks of the adjoint need.  count = 0 clears it. */
+ * finite-difference checks of the adjoint need.  count = 0 clears it.  The
+ * schedule survives lbm_upload (and the other calls that reset the fields),
+ * which rewind it to its first entry; lbm_adjoint neither uses nor changes
+ * it. */
 int lbm_set_iteration_schedule(void* ctx, const uint64_t* iterations, int64_t count);
 
 /* Poisson iterations of every step since the last upload (the CPU solver's
@@ -222,6 +261,14 @@ int lbm_download_dists(void* ctx, double* f, double* g, double* h);
 /* getLastPoissonDiagnostics (GPU/InamuroCUDA.cu:1714-1717). */
 int lbm_poisson_diag(const void* ctx, lbm_poisson_diag_t* out);
 
+/* Per-check records, the data behind setPoissonDiagnosticsPath
+ * (GPU/InamuroCUDA.hpp:87, GPU/InamuroCUDA.cu:2610-2682): fn(user, check) is
+ * called on the host after every completed convergence check whose values are
+ * finite (the reference writes its row at the same point), in order, from
+ * inside lbm_step.  The check's device work has finished when it is called.
+ * The record is valid during the call only.  NULL switches it off. */
+int lbm_set_check_observer(void* ctx, lbm_check_fn fn, void* user);
+
 /* synchronize (GPU/InamuroCUDA.cu:1709-1712). */
 int lbm_synchronize(void* ctx);
 
@@ -234,6 +281,10 @@ int lbm_perf_reset(void* ctx);
 int lbm_mass(void* ctx, double* sum_phi);
 
 /* Discrete adjoint (no counterpart in the reference; FP64 contexts only).
+ * On every return, successful or not, the iteration schedule of
+ * lbm_set_iteration_schedule and its position, event timing and the check
+ * observer are as they were before the call; no check records are reported
+ * while it runs.
  * Objective J = sum (phi_T - target)^2 (or lbm_set_adjoint_objective's
  * centroid height) after `steps` steps from phi0 with
  * rho0 = rho_G + (phi0 - fei_min)(rho_L - rho_G)/(fei_max - fei_min) (the

@@ -105,6 +105,8 @@ struct Ctx {
     int64_t adj_store_bytes = 0;     // device bytes for recorded Poisson iterates (0 = automatic)
     int adj_objective = lbm::ADJ_OBJ_MISFIT;
     bool adj_state_f32 = false;      // adjoint Poisson carriers stored in FP32
+    lbm_check_fn check_observer = nullptr;  // per-check records (Poisson diagnostics CSV)
+    void* check_user = nullptr;
     lbm_progress_fn progress = nullptr;  // lbm_adjoint progress reports
     void* progress_user = nullptr;
     lbm_adjoint_stats_t adj_stats{};
@@ -283,9 +285,10 @@ Ctx* create(const lbm_params_t& P, int precision)
             *a = dalloc_bytes(fb);
         c->p_prev = dalloc(n);
         if (c->mixed) c->scratch1 = dalloc(n);
-        const size_t nparts = std::max<size_t>(2 * static_cast<size_t>(std::max(lbm::grid_blocks(c->g.n),
-                                                                                lbm::stencil_blocks(c->g))),
-                                               9 * static_cast<size_t>(lbm::RED_BLOCKS));
+        const size_t nparts = std::max<size_t>(
+            std::max<size_t>(2 * static_cast<size_t>(lbm::grid_blocks(c->g.n)),
+                             lbm::SOURCE_TERMS * static_cast<size_t>(lbm::stencil_blocks(c->g))),
+            9 * static_cast<size_t>(lbm::RED_BLOCKS));
         c->partials = dalloc(nparts);
         c->dst = static_cast<DevState*>(dalloc_bytes(sizeof(DevState)));
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hst), sizeof(DevState), hipHostMallocDefault));
@@ -374,11 +377,12 @@ void step_fg(Ctx& c)
         lbm::f64::macro(s, 1, g, c.ph, D<double>(c.f[c.fcur]), D<double>(c.gd[c.fcur]),
                         D<double>(c.ux), D<double>(c.vy), D<double>(c.wz), c.dst, D<double>(c.fei),
                         D<double>(c.rho), D<double>(c.u), D<double>(c.v), D<double>(c.w),
-                        D<double>(c.tau), D<double>(c.src));
+                        D<double>(c.tau), D<double>(c.src), c.partials);
     } else {
         lbm::mx::macro(s, g, c.phf, D<float>(c.f[c.fcur]), D<float>(c.gd[c.fcur]), D<float>(c.ux),
                        D<float>(c.vy), D<float>(c.wz), c.dst, D<float>(c.fei), D<float>(c.rho),
-                       D<float>(c.u), D<float>(c.v), D<float>(c.w), D<float>(c.tau), D<float>(c.src));
+                       D<float>(c.u), D<float>(c.v), D<float>(c.w), D<float>(c.tau), D<float>(c.src),
+                       c.partials);
     }
     HIPCHK(hipGetLastError());
     record(c, 2);
@@ -544,7 +548,10 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
         d.pressure_l1_delta = st.e1;
         d.pressure_l1_norm = st.e2;
         d.divergence_mean = st.div_mean;
+        // GPU/InamuroCUDA.cu:1891-1893, 1920.
         d.source_mean = -st.div_mean / 3.0;
+        d.source_abs_mean = st.div_abs_sum / (3.0 * c.ph.cell_count);
+        d.projected_source_mean = -st.proj_div_sum / (3.0 * c.ph.cell_count);
         d.pressure_gauge_target = st.gauge_target;
         d.pressure_gauge_max_shift = st.max_shift;
         d.pressure_converged = st.pressure_converged;
@@ -561,6 +568,28 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
         }
         d.converged = st.converged;
         if (st.status != lbm::ST_OK) throw ApiError{LBM_ERR_NUMERIC, status_message(st.status)};
+        if (c.check_observer) {
+            // check_residual's row, GPU/InamuroCUDA.cu:2136-2147.
+            lbm_poisson_check_t rec{};
+            rec.step = c.perf.steps + 1;
+            rec.iteration = iter;
+            rec.pressure_l1_delta = d.pressure_l1_delta;
+            rec.pressure_l1_norm = d.pressure_l1_norm;
+            rec.relative_error = d.relative_error;
+            rec.fixed_point_h_l1 = d.fixed_point_h_l1;
+            rec.fixed_point_h_scale = d.fixed_point_h_scale;
+            rec.fixed_point_h_relative = d.fixed_point_h_relative;
+            rec.fixed_point_p_l1 = d.fixed_point_p_l1;
+            rec.fixed_point_p_scale = d.fixed_point_p_scale;
+            rec.fixed_point_p_relative = d.fixed_point_p_relative;
+            rec.fixed_point_relative = d.fixed_point_relative;
+            rec.fixed_point_evaluated = d.fixed_point_evaluated;
+            rec.pressure_converged = d.pressure_converged;
+            rec.fixed_point_converged = d.fixed_point_converged;
+            rec.dual_residual_enabled = d.dual_residual_enabled;
+            rec.converged = d.converged;
+            c.check_observer(c.check_user, &rec);
+        }
         if (hook) hook->fn(c, iter, hook->arg);
         if (forced != 0) {
             if (iter >= forced) break;
@@ -570,6 +599,7 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
         if (st.converged) break;
         if (c.iter_limit != 0 && iter >= c.iter_limit) {
             d.hit_max_iterations = 1;
+            d.fallback_count = 1;  // GPU/InamuroCUDA.cu:2335-2338
             break;
         }
         hmode = lbm::H_GAUGED;
@@ -1101,6 +1131,42 @@ void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*
     std::swap(a.lp, a.lps);
 }
 
+// Settings lbm_adjoint changes while it runs, put back on every exit: the
+// caller's iteration schedule and its position, event timing, the check
+// observer (the adjoint's recomputations are not steps of the simulation);
+// and, unless the forward pass completed, the performance totals and the
+// step history.
+struct AdjointGuard {
+    Ctx& c;
+    std::vector<uint64_t> schedule;
+    size_t sched_pos;
+    bool timing;
+    lbm_check_fn observer;
+    void* observer_user;
+    lbm_perf_t perf;
+    std::vector<uint64_t> history;
+
+    explicit AdjointGuard(Ctx& ctx)
+        : c(ctx), schedule(ctx.schedule), sched_pos(ctx.sched_pos), timing(ctx.timing),
+          observer(ctx.check_observer), observer_user(ctx.check_user), perf(ctx.perf),
+          history(ctx.history)
+    {
+        c.check_observer = nullptr;
+    }
+    ~AdjointGuard()
+    {
+        c.schedule = std::move(schedule);
+        c.sched_pos = sched_pos;
+        c.timing = timing;
+        c.check_observer = observer;
+        c.check_user = observer_user;
+        c.perf = perf;
+        c.history = std::move(history);
+    }
+    AdjointGuard(const AdjointGuard&) = delete;
+    AdjointGuard& operator=(const AdjointGuard&) = delete;
+};
+
 void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K, const uint64_t* iters,
                  double* J, double* grad_phi0, double* grad_kg)
 {
@@ -1201,6 +1267,8 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     a.have_snaps.assign(static_cast<size_t>(T), 0);
     a.pool_mark.assign(static_cast<size_t>(T), 0);
 
+    AdjointGuard guard(c);
+
     // Initial state: rho0 linear in phi0 (place_droplet's rule), rest zero.
     HostFields h0;
     h0.fei.assign(phi0, phi0 + n);
@@ -1259,8 +1327,11 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     auto t1 = std::chrono::steady_clock::now();
     stats.forward_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     const std::vector<uint64_t> sched(c.history.end() - T, c.history.end());
-    const std::vector<uint64_t> history = c.history;
     for (uint64_t v : sched) stats.forward_iterations += v;
+    // The forward pass's steps stay on the record; the replays below are not
+    // steps of the simulation.
+    guard.history = c.history;
+    guard.perf = c.perf;
 
     // Seed: J and dJ/dfei_T; every other adjoint starts at zero.
     if (misfit) HIPCHK(hipMemcpyAsync(a.target, target, n * sizeof(double), hipMemcpyHostToDevice, s));
@@ -1270,8 +1341,6 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     HIPCHK(hipMemsetAsync(a.cg[0], 0, slot_d * sizeof(double), s));
     HIPCHK(hipGetLastError());
 
-    const bool timing = c.timing;
-    const lbm_perf_t perf = c.perf;  // replays are not steps of the simulation
     c.timing = false;
     c.schedule = sched;
     int ccur = 0;
@@ -1317,14 +1386,10 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
     *J = a.has->J;
     if (grad_kg) *grad_kg = a.has->kg;
 
-    // Leave the context at the end of the forward pass.
+    // Leave the context at the end of the forward pass (the guard puts back
+    // the settings).
     restore_state(c, a.final_state);
     sync(c);
-    c.timing = timing;
-    c.perf = perf;
-    c.schedule.clear();
-    c.sched_pos = 0;
-    c.history = history;
     c.adj_stats = stats;
 }
 
@@ -1499,13 +1564,18 @@ int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance, uint64_
         if (!(tolerance > 0.0)) throw ApiError{LBM_ERR_ARG, "poisson tolerance must be positive"};
         if (iteration_limit != 0 && iteration_limit % static_cast<uint64_t>(check_interval) != 0)
             throw ApiError{LBM_ERR_ARG, "Poisson iteration limit must be a multiple of check interval"};
-        c.check_interval = check_interval;
-        c.tol = tolerance;
-        c.iter_limit = iteration_limit;
         if (dual_residual < 0 || dual_residual > 2)
             throw ApiError{LBM_ERR_ARG, "dual_residual must be 0 (off), 1 (dual exit) or 2 (shadow)"};
         if (dual_residual != 0 && c.g.zwall)
             throw ApiError{LBM_ERR_ARG, "the dual residual covers the reference boundaries only (no z walls)"};
+        for (uint64_t it : c.schedule)
+            if (it % static_cast<uint64_t>(check_interval) != 0)
+                throw ApiError{LBM_ERR_ARG,
+                               "the iteration schedule holds counts that are not multiples of this check interval"};
+        // Every argument is valid: only now does the context change.
+        c.check_interval = check_interval;
+        c.tol = tolerance;
+        c.iter_limit = iteration_limit;
         c.dual = dual_residual;
         clear_graphs(c);
     });
@@ -1635,6 +1705,14 @@ int lbm_poisson_diag(const void* ctx, lbm_poisson_diag_t* out)
     if (!ctx || !out) return LBM_ERR_ARG;
     *out = static_cast<const Ctx*>(ctx)->diag;
     return LBM_OK;
+}
+
+int lbm_set_check_observer(void* ctx, lbm_check_fn fn, void* user)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        c.check_observer = fn;
+        c.check_user = user;
+    });
 }
 
 int lbm_synchronize(void* ctx)

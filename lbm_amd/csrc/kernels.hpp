@@ -32,8 +32,14 @@ enum : int32_t {
     ST_FP_REL_NONFINITE = 6,   // "normalized kinetic fixed-point residual contains NaN/Inf"
 };
 
+// Per-block terms of the collision kernels' source reduction: sum div,
+// sum p, sum |div|.
+constexpr int SOURCE_TERMS = 3;
+
 struct DevState {
     double div_mean;
+    double div_abs_sum;    // sum |u_x + v_y + w_z| before the projection
+    double proj_div_sum;   // sum of the projected source
     double gauge_target;
     double shift;          // gauge shift of the latest check
     double max_shift;
@@ -83,10 +89,13 @@ void collide(hipStream_t s, int mode, const Grid& g, const Phys& ph,
              double* ux, double* vy, double* wz, double* partials);
 void source_finalize(hipStream_t s, const Grid& g, const Phys& ph, const double* partials,
                      int nparts, DevState* st);
+// Also reduces the projected source into st->proj_div_sum; partials holds
+// grid_blocks(g.n) doubles.
 void macro(hipStream_t s, int pull, const Grid& g, const Phys& ph, const double* fin,
            const double* gin, const double* ux, const double* vy, const double* wz,
-           const DevState* st, double* fei, double* rho, double* u, double* v, double* w,
-           double* tau, double* src);
+           DevState* st, double* fei, double* rho, double* u, double* v, double* w,
+           double* tau, double* src, double* partials);
+void projected_finalize(hipStream_t s, const double* parts, int nparts, DevState* st);
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, double* hout,
                   const double* p, const double* tau, const double* src, const DevState* st,
                   bool nontemporal);
@@ -106,6 +115,9 @@ void correct(hipStream_t s, const Grid& g, const double* p, const double* rho, d
 void materialize(hipStream_t s, const Grid& g, const double* in, double* out);
 void equilibrium_h(hipStream_t s, const Grid& g, const double* p, double* h);
 void sum_field(hipStream_t s, const Grid& g, const double* a, double* partials, DevState* st);
+// The same deterministic sum of a[0..g.n) into the device double *out;
+// partials holds red_blocks(g.n) <= RED_BLOCKS doubles.  Kernel launches only.
+void sum_field_into(hipStream_t s, const Grid& g, const double* a, double* partials, double* out);
 // getp + gauge of a check without the convergence test (recomputation of an
 // already converged solve): p = sum_q view(h), shift, p -= shift.
 void gauge(hipStream_t s, const Grid& g, const Phys& ph, const double* hin, double* p,
@@ -181,8 +193,8 @@ void collide(hipStream_t s, int mode, const Grid& g, const PhysF& ph, const floa
              const float* fin, const float* gin, float* fout, float* gout, float* ux, float* vy,
              float* wz, double* partials);
 void macro(hipStream_t s, const Grid& g, const PhysF& ph, float* fpost, const float* gpost,
-           const float* ux, const float* vy, const float* wz, const DevState* st, float* fei,
-           float* rho, float* u, float* v, float* w, float* omega, float* src3);
+           const float* ux, const float* vy, const float* wz, DevState* st, float* fei,
+           float* rho, float* u, float* v, float* w, float* omega, float* src3, double* partials);
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const float* hin, float* hout,
                   const float* bbase, const double* pown, float* bout, const float* omega,
                   const float* src3, const DevState* st);

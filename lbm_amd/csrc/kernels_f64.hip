@@ -139,7 +139,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
     const int tx = threadIdx.x % STENCIL_TX, ty = threadIdx.x / STENCIL_TX;
     const int x = x0 + tx, y = y0 + ty;
     const bool valid = x < g.lx && y < g.ly;
-    double div_c = 0.0, p_c = 0.0;
+    double div_c = 0.0, p_c = 0.0, adiv_c = 0.0;
     for (int z = zb; z < ze; ++z) {
         const int sm = (z - zb) % 3, s0 = (z - zb + 1) % 3, sp = (z - zb + 2) % 3;
         land(sp);
@@ -237,16 +237,20 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
             ux_o[c] = u_x;
             vy_o[c] = v_y;
             wz_o[c] = w_z;
-            div_c += u_x + v_y + w_z;
+            const double div = u_x + v_y + w_z;
+            div_c += div;
+            adiv_c += fabs(div);
             p_c += p[c];
         }
         __syncthreads();  // the next layer's staging reuses the oldest slot
     }
     const double sd = block_sum(div_c, lds);
     const double sp = block_sum(p_c, lds);
+    const double sa = block_sum(adiv_c, lds);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = sd;
-        partials[2 * blockIdx.x + 1] = sp;
+        partials[SOURCE_TERMS * blockIdx.x] = sd;
+        partials[SOURCE_TERMS * blockIdx.x + 1] = sp;
+        partials[SOURCE_TERMS * blockIdx.x + 2] = sa;
     }
 }
 
@@ -259,17 +263,20 @@ __device__ __forceinline__ double reduce_partials(const double* __restrict__ par
     return block_sum(acc, lds);
 }
 
-// Source statistics of solvePressurePoisson, CPU/Inamuro.cpp:652-664.
+// Source statistics of solvePressurePoisson, CPU/Inamuro.cpp:652-664, with
+// the GPU solver's sum |div| (GPU/InamuroCUDA.cu:1868-1893).
 __global__ __launch_bounds__(BLOCK) void k_source_finalize(Phys ph, const double* __restrict__ parts,
                                                            int nparts, DevState* st)
 {
     __shared__ double lds[4];
-    const double sd = reduce_partials(parts, nparts, 2, 0, lds);
-    const double sp = reduce_partials(parts, nparts, 2, 1, lds);
+    const double sd = reduce_partials(parts, nparts, SOURCE_TERMS, 0, lds);
+    const double sp = reduce_partials(parts, nparts, SOURCE_TERMS, 1, lds);
+    const double sa = reduce_partials(parts, nparts, SOURCE_TERMS, 2, lds);
     if (threadIdx.x == 0) {
         const double dm = sd / ph.cell_count;
         const double gt = sp / ph.cell_count;
         st->div_mean = dm;
+        st->div_abs_sum = sa;
         st->gauge_target = gt;
         st->shift = 0.0;
         st->max_shift = 0.0;
@@ -288,16 +295,14 @@ __global__ __launch_bounds__(BLOCK) void k_source_finalize(Phys ph, const double
 //   tau_h = 1.0/rho + 0.5                           (CPU/Inamuro.cpp:764)
 //   S     = ((u_x - mean) + v_y) + w_z              (CPU/Inamuro.cpp:665-668, 773)
 template <int PULL>
-__global__ __launch_bounds__(BLOCK) void k_macro(
-    Grid g, Phys ph, const double* __restrict__ fin, const double* __restrict__ gin,
-    const double* __restrict__ ux, const double* __restrict__ vy, const double* __restrict__ wz,
-    const DevState* __restrict__ st,
+__device__ __forceinline__ double macro_cell(
+    int64_t c, const Grid& g, const Phys& ph, const double* __restrict__ fin,
+    const double* __restrict__ gin, const double* __restrict__ ux, const double* __restrict__ vy,
+    const double* __restrict__ wz, const DevState* __restrict__ st,
     double* __restrict__ fei, double* __restrict__ rho, double* __restrict__ u,
     double* __restrict__ v, double* __restrict__ w, double* __restrict__ tau,
     double* __restrict__ src)
 {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
-    if (c >= g.n) return;
     int x, y, z;
     cell_xyz(c, g, x, y, z);
     double fv[Q], gv[Q];
@@ -336,7 +341,27 @@ __global__ __launch_bounds__(BLOCK) void k_macro(
     v[c] = vv;
     w[c] = ww;
     tau[c] = 1.0 / density + 0.5;
-    src[c] = ((ux[c] - st->div_mean) + vy[c]) + wz[c];
+    const double sc = ((ux[c] - st->div_mean) + vy[c]) + wz[c];
+    src[c] = sc;
+    return sc;
+}
+
+template <int PULL>
+__global__ __launch_bounds__(BLOCK) void k_macro(
+    Grid g, Phys ph, const double* __restrict__ fin, const double* __restrict__ gin,
+    const double* __restrict__ ux, const double* __restrict__ vy, const double* __restrict__ wz,
+    const DevState* __restrict__ st,
+    double* __restrict__ fei, double* __restrict__ rho, double* __restrict__ u,
+    double* __restrict__ v, double* __restrict__ w, double* __restrict__ tau,
+    double* __restrict__ src, double* __restrict__ parts)
+{
+    __shared__ double lds[4];
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    double s_c = 0.0;
+    if (c < g.n) s_c = macro_cell<PULL>(c, g, ph, fin, gin, ux, vy, wz, st, fei, rho, u, v, w, tau, src);
+    // Block sums of the projected source (GPU/InamuroCUDA.cu:1897-1920).
+    const double bs = block_sum(s_c, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = bs;
 }
 
 // Loads the 15 streamed populations of cell c and applies the x-slip rule
@@ -690,12 +715,23 @@ __global__ __launch_bounds__(BLOCK) void k_equilibrium_h(Grid g, const double* _
     for (int q = 0; q < Q; ++q) h[q * g.n + c] = pc * EW[q];
 }
 
-__global__ __launch_bounds__(BLOCK) void k_mass_finalize(const double* __restrict__ parts,
-                                                         int nparts, DevState* st)
+// Sum of the projected source over the grid, from k_macro's block sums
+// (GPU/InamuroCUDA.cu:1897-1920).
+__global__ __launch_bounds__(BLOCK) void k_projected_finalize(const double* __restrict__ parts,
+                                                              int nparts, DevState* st)
 {
     __shared__ double lds[4];
     const double s = reduce_partials(parts, nparts, 1, 0, lds);
-    if (threadIdx.x == 0) st->mass = s;
+    if (threadIdx.x == 0) st->proj_div_sum = s;
+}
+
+// Fixed-order total of nparts block sums into *out.
+__global__ __launch_bounds__(BLOCK) void k_sum_finalize(const double* __restrict__ parts, int nparts,
+                                                        double* __restrict__ out)
+{
+    __shared__ double lds[4];
+    const double s = reduce_partials(parts, nparts, 1, 0, lds);
+    if (threadIdx.x == 0) *out = s;
 }
 
 } // namespace
@@ -730,14 +766,22 @@ void source_finalize(hipStream_t s, const Grid&, const Phys& ph, const double* p
 
 void macro(hipStream_t s, int pull, const Grid& g, const Phys& ph, const double* fin,
            const double* gin, const double* ux, const double* vy, const double* wz,
-           const DevState* st, double* fei, double* rho, double* u, double* v, double* w,
-           double* tau, double* src)
+           DevState* st, double* fei, double* rho, double* u, double* v, double* w,
+           double* tau, double* src, double* partials)
 {
     const int nb = grid_blocks(g.n);
     if (pull)
-        k_macro<1><<<nb, BLOCK, 0, s>>>(g, ph, fin, gin, ux, vy, wz, st, fei, rho, u, v, w, tau, src);
+        k_macro<1><<<nb, BLOCK, 0, s>>>(g, ph, fin, gin, ux, vy, wz, st, fei, rho, u, v, w, tau, src,
+                                        partials);
     else
-        k_macro<0><<<nb, BLOCK, 0, s>>>(g, ph, fin, gin, ux, vy, wz, st, fei, rho, u, v, w, tau, src);
+        k_macro<0><<<nb, BLOCK, 0, s>>>(g, ph, fin, gin, ux, vy, wz, st, fei, rho, u, v, w, tau, src,
+                                        partials);
+    k_projected_finalize<<<1, BLOCK, 0, s>>>(partials, nb, st);
+}
+
+void projected_finalize(hipStream_t s, const double* parts, int nparts, DevState* st)
+{
+    k_projected_finalize<<<1, BLOCK, 0, s>>>(parts, nparts, st);
 }
 
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, double* hout,
@@ -806,7 +850,7 @@ void check_finalize(hipStream_t s, const double* parts_err, const double* parts_
 
 void mass_finalize(hipStream_t s, const double* parts, int nparts, DevState* st)
 {
-    k_mass_finalize<<<1, BLOCK, 0, s>>>(parts, nparts, st);
+    k_sum_finalize<<<1, BLOCK, 0, s>>>(parts, nparts, &st->mass);
 }
 
 void streamed_to_post_boundary(hipStream_t s, const Grid& g, const double* hin, double* out)
@@ -834,7 +878,14 @@ void sum_field(hipStream_t s, const Grid& g, const double* a, double* partials, 
 {
     const int nb = red_blocks(g.n);
     k_sum_strided<<<nb, BLOCK, 0, s>>>(g, a, partials);
-    k_mass_finalize<<<1, BLOCK, 0, s>>>(partials, nb, st);
+    k_sum_finalize<<<1, BLOCK, 0, s>>>(partials, nb, &st->mass);
+}
+
+void sum_field_into(hipStream_t s, const Grid& g, const double* a, double* partials, double* out)
+{
+    const int nb = red_blocks(g.n);
+    k_sum_strided<<<nb, BLOCK, 0, s>>>(g, a, partials);
+    k_sum_finalize<<<1, BLOCK, 0, s>>>(partials, nb, out);
 }
 
 } // namespace f64

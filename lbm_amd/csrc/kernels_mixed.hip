@@ -203,7 +203,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
     const int tx = threadIdx.x % STENCIL_TX, ty = threadIdx.x / STENCIL_TX;
     const int x = x0 + tx, y = y0 + ty;
     const bool valid = x < g.lx && y < g.ly;
-    double div_c = 0.0, p_c = 0.0;
+    double div_c = 0.0, p_c = 0.0, adiv_c = 0.0;
     for (int z = zb; z < ze; ++z) {
         const int sm = (z - zb) % 3, s0 = (z - zb + 1) % 3, sp = (z - zb + 2) % 3;
         stage(z_stencil(z, 1, g), sp);
@@ -268,30 +268,33 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
             ux_o[c] = U.gx;
             vy_o[c] = V.gy;
             wz_o[c] = W.gz;
-            div_c += static_cast<double>(U.gx) + static_cast<double>(V.gy) + static_cast<double>(W.gz);
+            const double div = static_cast<double>(U.gx) + static_cast<double>(V.gy) +
+                               static_cast<double>(W.gz);
+            div_c += div;
+            adiv_c += fabs(div);
             p_c += p[c];
         }
         __syncthreads();  // the next layer's staging reuses the oldest slot
     }
     const double sd = block_sum(div_c, lds);
     const double sp = block_sum(p_c, lds);
+    const double sa = block_sum(adiv_c, lds);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = sd;
-        partials[2 * blockIdx.x + 1] = sp;
+        partials[SOURCE_TERMS * blockIdx.x] = sd;
+        partials[SOURCE_TERMS * blockIdx.x + 1] = sp;
+        partials[SOURCE_TERMS * blockIdx.x + 2] = sa;
     }
 }
 
 // getMacro (CPU/Inamuro.cpp:294-346) on the pulled deviations; re-bases f'_0
 // on the new phi; writes omega = 1 - 1/tau_h and S/3 for the Poisson solve.
-__global__ __launch_bounds__(BLOCK) void k_macro(
-    Grid g, PhysF ph, float* __restrict__ fpost, const float* __restrict__ gpost,
+__device__ __forceinline__ double macro_cell(
+    int64_t c, const Grid& g, const PhysF& ph, float* __restrict__ fpost, const float* __restrict__ gpost,
     const float* __restrict__ ux, const float* __restrict__ vy, const float* __restrict__ wz,
     const DevState* __restrict__ st, float* __restrict__ fei, float* __restrict__ rho,
     float* __restrict__ u, float* __restrict__ v, float* __restrict__ w,
     float* __restrict__ omega, float* __restrict__ src3)
 {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
-    if (c >= g.n) return;
     int x, y, z;
     cell_xyz(c, g, x, y, z);
     float fv[Q], gv[Q];
@@ -327,7 +330,25 @@ __global__ __launch_bounds__(BLOCK) void k_macro(
     const float tauh = 1.0f / density + 0.5f;
     omega[c] = 1.0f - 1.0f / tauh;
     const float m = static_cast<float>(st->div_mean);
-    src3[c] = (((ux[c] - m) + vy[c]) + wz[c]) * (1.0f / 3.0f);
+    const float sc = ((ux[c] - m) + vy[c]) + wz[c];
+    src3[c] = sc * (1.0f / 3.0f);
+    return static_cast<double>(sc);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_macro(
+    Grid g, PhysF ph, float* __restrict__ fpost, const float* __restrict__ gpost,
+    const float* __restrict__ ux, const float* __restrict__ vy, const float* __restrict__ wz,
+    const DevState* __restrict__ st, float* __restrict__ fei, float* __restrict__ rho,
+    float* __restrict__ u, float* __restrict__ v, float* __restrict__ w,
+    float* __restrict__ omega, float* __restrict__ src3, double* __restrict__ parts)
+{
+    __shared__ double lds[4];
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    double s_c = 0.0;
+    if (c < g.n) s_c = macro_cell(c, g, ph, fpost, gpost, ux, vy, wz, st, fei, rho, u, v, w, omega, src3);
+    // Block sums of the projected source (GPU/InamuroCUDA.cu:1897-1920).
+    const double bs = block_sum(s_c, lds);
+    if (threadIdx.x == 0) parts[blockIdx.x] = bs;
 }
 
 // Reconstructs the post-boundary FP64 pressure populations of cell c from the
@@ -626,11 +647,13 @@ void collide(hipStream_t s, int mode, const Grid& g, const PhysF& ph, const floa
 }
 
 void macro(hipStream_t s, const Grid& g, const PhysF& ph, float* fpost, const float* gpost,
-           const float* ux, const float* vy, const float* wz, const DevState* st, float* fei,
-           float* rho, float* u, float* v, float* w, float* omega, float* src3)
+           const float* ux, const float* vy, const float* wz, DevState* st, float* fei,
+           float* rho, float* u, float* v, float* w, float* omega, float* src3, double* partials)
 {
-    k_macro<<<grid_blocks(g.n), BLOCK, 0, s>>>(g, ph, fpost, gpost, ux, vy, wz, st, fei, rho, u, v, w,
-                                               omega, src3);
+    const int nb = grid_blocks(g.n);
+    k_macro<<<nb, BLOCK, 0, s>>>(g, ph, fpost, gpost, ux, vy, wz, st, fei, rho, u, v, w, omega, src3,
+                                 partials);
+    f64::projected_finalize(s, partials, nb, st);
 }
 
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const float* hin, float* hout,

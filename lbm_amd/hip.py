@@ -44,7 +44,7 @@ EXPORTED = (
     "lbm_reset_fields", "lbm_set_iteration_schedule", "lbm_iteration_history",
     "lbm_adjoint", "lbm_set_adjoint_memory", "lbm_adjoint_stats", "lbm_set_poisson_graph",
     "lbm_set_z_walls", "lbm_set_gravity", "lbm_set_adjoint_objective",
-    "lbm_set_adjoint_state_precision", "lbm_set_progress",
+    "lbm_set_adjoint_state_precision", "lbm_set_progress", "lbm_set_check_observer",
 )
 
 # lbm_progress_fn: (user, phase, step, iterations)
@@ -74,12 +74,33 @@ class PoissonStepDiagnostics(ctypes.Structure):
         ("fixed_point_h_scale", ctypes.c_double), ("fixed_point_h_relative", ctypes.c_double),
         ("fixed_point_p_l1", ctypes.c_double), ("fixed_point_p_scale", ctypes.c_double),
         ("fixed_point_p_relative", ctypes.c_double), ("fixed_point_relative", ctypes.c_double),
-        ("source_mean", ctypes.c_double), ("divergence_mean", ctypes.c_double),
+        ("source_mean", ctypes.c_double), ("source_abs_mean", ctypes.c_double),
+        ("projected_source_mean", ctypes.c_double), ("divergence_mean", ctypes.c_double),
         ("pressure_gauge_target", ctypes.c_double), ("pressure_gauge_max_shift", ctypes.c_double),
         ("fixed_point_evaluated", ctypes.c_int32), ("pressure_converged", ctypes.c_int32),
         ("fixed_point_converged", ctypes.c_int32), ("dual_residual_enabled", ctypes.c_int32),
-        ("converged", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("converged", ctypes.c_int32), ("fallback_count", ctypes.c_int32),
     ]
+
+
+class PoissonCheck(ctypes.Structure):
+    """lbm_poisson_check_t: one convergence check, the values of one row of
+    the reference's Poisson diagnostics CSV (GPU/InamuroCUDA.cu:2634-2682)."""
+    _fields_ = [
+        ("step", ctypes.c_int64), ("iteration", ctypes.c_uint64),
+        ("pressure_l1_delta", ctypes.c_double), ("pressure_l1_norm", ctypes.c_double),
+        ("relative_error", ctypes.c_double), ("fixed_point_h_l1", ctypes.c_double),
+        ("fixed_point_h_scale", ctypes.c_double), ("fixed_point_h_relative", ctypes.c_double),
+        ("fixed_point_p_l1", ctypes.c_double), ("fixed_point_p_scale", ctypes.c_double),
+        ("fixed_point_p_relative", ctypes.c_double), ("fixed_point_relative", ctypes.c_double),
+        ("fixed_point_evaluated", ctypes.c_int32), ("pressure_converged", ctypes.c_int32),
+        ("fixed_point_converged", ctypes.c_int32), ("dual_residual_enabled", ctypes.c_int32),
+        ("converged", ctypes.c_int32), ("pad", ctypes.c_int32),
+    ]
+
+
+# lbm_check_fn: (user, const lbm_poisson_check_t*)
+CHECK_FN = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(PoissonCheck))
 
 
 class PerformanceMetrics(ctypes.Structure):
@@ -148,6 +169,7 @@ def load_library():
     L.lbm_set_adjoint_state_precision.argtypes = [vp, ctypes.c_int32]
     L.lbm_set_progress.argtypes = [vp, PROGRESS_FN, vp]
     L.lbm_adjoint_stats.argtypes = [vp, ctypes.POINTER(AdjointStats)]
+    L.lbm_set_check_observer.argtypes = [vp, CHECK_FN, vp]
     _lib = L
     return L
 
@@ -348,6 +370,21 @@ class InamuroHIP:
         d = PoissonStepDiagnostics()
         self._check(load_library().lbm_poisson_diag(self._h, ctypes.byref(d)))
         return d
+
+    def set_check_observer(self, fn):
+        """fn(record) after every completed Poisson convergence check, with
+        a dict of the lbm_poisson_check_t fields; None switches it off."""
+        if fn is None:
+            self._check_cb = None
+            self._check(load_library().lbm_set_check_observer(self._h, CHECK_FN(), None))
+            return
+
+        def cb(_u, rec):
+            r = rec.contents
+            fn({k: getattr(r, k) for k, _ in PoissonCheck._fields_ if k != "pad"})
+
+        self._check_cb = CHECK_FN(cb)
+        self._check(load_library().lbm_set_check_observer(self._h, self._check_cb, None))
 
     def getGridSize(self):
         return (self._cp.lx, self._cp.ly, self._cp.lz)

@@ -96,6 +96,54 @@ void InamuroHIP::setUsePoissonGraph(bool enabled)
     check(lbm_set_poisson_graph(ctx, enabled ? 1 : 0), ctx);
 }
 
+namespace {
+
+// One row of writePoissonDiagnostic (GPU/InamuroCUDA.cu:2652-2681) with
+// spatial diagnostics off: empty block_low_frequency_fraction, block size
+// and count 0.
+void write_check_row(void* user, const lbm_poisson_check_t* r)
+{
+    std::ofstream& out = *static_cast<std::ofstream*>(user);
+    out << r->step << ',' << r->iteration << ',' << std::setprecision(17) << r->pressure_l1_delta << ','
+        << r->pressure_l1_norm << ',' << r->relative_error << ',';
+    auto opt = [&](double v) {
+        if (std::isfinite(v)) out << v;
+        out << ',';
+    };
+    opt(r->fixed_point_h_l1);
+    opt(r->fixed_point_h_scale);
+    opt(r->fixed_point_h_relative);
+    opt(r->fixed_point_p_l1);
+    opt(r->fixed_point_p_scale);
+    opt(r->fixed_point_p_relative);
+    opt(r->fixed_point_relative);
+    out << (r->fixed_point_evaluated ? 1 : 0) << ',' << (r->pressure_converged ? 1 : 0) << ','
+        << (r->fixed_point_converged ? 1 : 0) << ',' << (r->dual_residual_enabled ? 1 : 0) << ','
+        << (r->converged ? 1 : 0) << ',';
+    out << ',' << 0 << ',' << 0 << '\n';
+    out.flush();
+}
+
+} // namespace
+
+void InamuroHIP::setPoissonDiagnosticsPath(const std::string& path)
+{
+    check(lbm_set_check_observer(ctx, nullptr, nullptr), ctx);
+    pdiag_.reset();
+    if (path.empty()) return;
+    auto out = std::make_unique<std::ofstream>(path);
+    if (!*out) throw std::runtime_error("failed to open Poisson diagnostics CSV: " + path);
+    *out << "step,iteration,pressure_l1_delta,pressure_l1_norm,relative_error,"
+         << "fixed_point_h_l1,fixed_point_h_scale,fixed_point_h_relative,"
+         << "fixed_point_p_l1,fixed_point_p_scale,fixed_point_p_relative,"
+         << "fixed_point_relative,fixed_point_evaluated,pressure_converged,fixed_point_converged,"
+         << "dual_residual_enabled,converged,block_low_frequency_fraction,"
+         << "block_size,block_count\n";
+    out->flush();
+    check(lbm_set_check_observer(ctx, write_check_row, out.get()), ctx);
+    pdiag_ = std::move(out);
+}
+
 lbm_poisson_diag_t InamuroHIP::getLastPoissonDiagnostics() const
 {
     lbm_poisson_diag_t d;

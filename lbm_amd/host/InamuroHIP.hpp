@@ -11,7 +11,9 @@
 #pragma once
 
 #include <cstdint>
+#include <fstream>
 #include <limits>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -26,7 +28,12 @@ public:
     ~InamuroHIP();
     InamuroHIP(const InamuroHIP&) = delete;
     InamuroHIP& operator=(const InamuroHIP&) = delete;
-    InamuroHIP(InamuroHIP&& o) noexcept : ctx(o.ctx), n(o.n) { o.ctx = nullptr; }
+    InamuroHIP(InamuroHIP&& o) noexcept
+        : ctx(o.ctx), n(o.n), interval_(o.interval_), tol_(o.tol_), limit_(o.limit_), dual_(o.dual_),
+          pdiag_(std::move(o.pdiag_))
+    {
+        o.ctx = nullptr;
+    }
 
     void performTimeStepGPU();
     void synchronize() const;
@@ -37,6 +44,11 @@ public:
     // Segment replay as a captured HIP graph (GPU/InamuroCUDA.hpp:80); on by
     // default, results identical either way.
     void setUsePoissonGraph(bool enabled);
+    // setPoissonDiagnosticsPath (GPU/InamuroCUDA.cu:2610-2627): truncates the
+    // file, writes the 20-column header, then appends one row per
+    // convergence check (writePoissonDiagnostic, :2634-2682).  An empty path
+    // switches the file off.
+    void setPoissonDiagnosticsPath(const std::string& path);
     lbm_poisson_diag_t getLastPoissonDiagnostics() const;
     void downloadHH(std::vector<double>& hh) const;
     // fei, rho, u, v, w, p in the canonical x-fastest layout (each n values).
@@ -59,6 +71,7 @@ private:
     double tol_ = 1.0e-3;
     std::uint64_t limit_ = 0;
     int dual_ = 0;
+    std::unique_ptr<std::ofstream> pdiag_;  // Poisson diagnostics CSV (heap: stable for the callback)
 };
 
 class Inamuro : public LBMBase {

@@ -207,35 +207,13 @@ int main(int argc, char** argv)
         oc.output_dir = "out/";
         solver.setOutputConfig(oc);
 
-        std::ofstream pdiag;
-        if (!cfg.poisson_diagnostics.empty()) {
-            pdiag.open(cfg.poisson_diagnostics);
-            if (!pdiag) throw std::runtime_error("failed to open Poisson diagnostics CSV: " + cfg.poisson_diagnostics);
-            pdiag << "step,iteration,pressure_l1_delta,pressure_l1_norm,relative_error,"
-                  << "fixed_point_h_l1,fixed_point_h_scale,fixed_point_h_relative,"
-                  << "fixed_point_p_l1,fixed_point_p_scale,fixed_point_p_relative,"
-                  << "fixed_point_relative,fixed_point_evaluated,pressure_converged,fixed_point_converged,"
-                  << "dual_residual_enabled,converged\n";
-        }
+        // GPU/main.cpp:355: one row per convergence check.
+        gpu.setPoissonDiagnosticsPath(cfg.poisson_diagnostics);
         if (cfg.write_output && (cfg.output_steps.empty() || cfg.output_steps.count(0) > 0))
             solver.writeResults(0);
         const auto t0 = std::chrono::steady_clock::now();
         for (int step = 0; step < cfg.steps; ++step) {
             solver.performTimeStep();
-            if (pdiag) {
-                const lbm_poisson_diag_t d = gpu.getLastPoissonDiagnostics();
-                auto opt = [&](double v) {
-                    if (std::isfinite(v)) pdiag << v;
-                    pdiag << ',';
-                };
-                pdiag << step + 1 << ',' << d.iterations << ',' << std::setprecision(17) << d.pressure_l1_delta
-                      << ',' << d.pressure_l1_norm << ',' << d.relative_error << ',';
-                opt(d.fixed_point_h_l1); opt(d.fixed_point_h_scale); opt(d.fixed_point_h_relative);
-                opt(d.fixed_point_p_l1); opt(d.fixed_point_p_scale); opt(d.fixed_point_p_relative);
-                opt(d.fixed_point_relative);
-                pdiag << d.fixed_point_evaluated << ',' << d.pressure_converged << ',' << d.fixed_point_converged
-                      << ',' << d.dual_residual_enabled << ',' << d.converged << '\n';
-            }
             if (should_write(cfg, step + 1)) solver.writeResults(step + 1);
         }
         gpu.synchronize();

@@ -668,6 +668,11 @@ static int fixed_point_residual(orc_sim* s, double gauge_target)
     return 0;
 }
 
+int orc_fixed_point_residual(orc_sim* s, double gauge_target)
+{
+    return fixed_point_residual(s, gauge_target);
+}
+
 void orc_poisson_iterations(orc_sim* s, int k)
 {
     for (int it = 0; it < k; ++it) {
@@ -701,24 +706,33 @@ static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p)
     d->fixed_point_relative = NAN;
 
     const double cell_count = (double)s->lx * s->ly * s->lz;
-    double divergence_sum = 0.0, pressure_sum = 0.0;
+    double divergence_sum = 0.0, pressure_sum = 0.0, divergence_abs_sum = 0.0;
     double* ux = A(s, ORC_U_X);
     const double* vy = A(s, ORC_V_Y);
     const double* wz = A(s, ORC_W_Z);
     double* p = A(s, ORC_P);
     for (size_t i = 0; i < n; ++i) {
-        divergence_sum += ux[i] + vy[i] + wz[i];
+        const double div = ux[i] + vy[i] + wz[i];
+        divergence_sum += div;
+        divergence_abs_sum += fabs(div);
         pressure_sum += p[i];
     }
     const double divergence_mean = divergence_sum / cell_count;
     const double gauge_target = pressure_sum / cell_count;
     d->divergence_mean = divergence_mean;
     d->pressure_gauge_target = gauge_target;
+    /* GPU-only source statistics, GPU/InamuroCUDA.cu:1891-1893. */
+    d->source_mean = -divergence_mean / 3.0;
+    d->source_abs_mean = divergence_abs_sum / (3.0 * cell_count);
     if (!isfinite(divergence_mean) || !isfinite(gauge_target)) {
         set_err(s, "Poisson source/gauge contains NaN/Inf");
         return -1;
     }
     for (size_t i = 0; i < n; ++i) ux[i] -= divergence_mean;
+    /* GPU/InamuroCUDA.cu:1897-1920: the source's sum after the projection. */
+    double projected_sum = 0.0;
+    for (size_t i = 0; i < n; ++i) projected_sum += ux[i] + vy[i] + wz[i];
+    d->projected_source_mean = -projected_sum / (3.0 * cell_count);
 
     const uint64_t forced = s->schedule_pos < s->schedule_len ? s->schedule[s->schedule_pos] : 0;
     uint64_t iter = 0;
@@ -758,7 +772,11 @@ static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p)
                 continue;
             }
             if (error < s->tol && fp_ok) { d->converged = 1; break; }
-            if (s->iter_cap != 0 && iter >= s->iter_cap) { d->hit_max_iterations = 1; break; }
+            if (s->iter_cap != 0 && iter >= s->iter_cap) {
+                d->hit_max_iterations = 1;
+                d->fallback_count = 1;  /* GPU/InamuroCUDA.cu:2335-2338 */
+                break;
+            }
         }
     }
     s->last_iteration_count = iter;

@@ -56,6 +56,16 @@ typedef struct orc_poisson_diag {
     double fixed_point_relative;
     int32_t fixed_point_evaluated;
     int32_t converged;
+    /* GPU PoissonStepDiagnostics source statistics (GPU/InamuroCUDA.cu:
+     * 1891-1893, 1920), sums in x-outer order: -mean(div)/3, mean|div|/3 of
+     * the unprojected source and -mean(div)/3 after the projection. */
+    double source_mean;
+    double source_abs_mean;
+    double projected_source_mean;
+    /* 1 when the solve stopped at a nonzero iteration cap
+     * (GPU/InamuroCUDA.cu:2189, 2337). */
+    int32_t fallback_count;
+    int32_t reserved;
 } orc_poisson_diag;
 
 /* Array identifiers for orc_get / orc_set. */
@@ -161,6 +171,14 @@ void orc_set_gravity(orc_sim* s, double gz);
 /* Closed-form helpers exposed for unit tests. */
 double orc_eos(const orc_params* p, double fei);                 /* CPU/Inamuro.cpp:843-847 */
 double orc_rho_of_phi(const orc_params* p, double fei);           /* CPU/Inamuro.cpp:313-327 */
+
+/* The kinetic fixed-point residual (DUAL_RESIDUAL_CONTRACT.md:24-41,
+ * GPU/InamuroCUDA.cu:1987-2057, 1114-1171) of the current h, p, rho and
+ * source (u_x + v_y + w_z) against the gauge target g, exactly as a dual-mode
+ * check evaluates it after its gauge: the six sums and three ratios go to the
+ * fixed_point_* fields of the diagnostics.  Exposed so tests can evaluate it
+ * on states of their own.  0 = ok. */
+int orc_fixed_point_residual(orc_sim* s, double gauge_target);
 
 /* Timing helper for the CPU baseline: runs k bare Poisson iterations
  * (collision_p, stream, BC, getp) with no convergence test. */

@@ -94,7 +94,9 @@ class PoissonDiag(ctypes.Structure):
         ("fixed_point_h_relative", ctypes.c_double), ("fixed_point_p_l1", ctypes.c_double),
         ("fixed_point_p_scale", ctypes.c_double), ("fixed_point_p_relative", ctypes.c_double),
         ("fixed_point_relative", ctypes.c_double), ("fixed_point_evaluated", ctypes.c_int32),
-        ("converged", ctypes.c_int32),
+        ("converged", ctypes.c_int32), ("source_mean", ctypes.c_double),
+        ("source_abs_mean", ctypes.c_double), ("projected_source_mean", ctypes.c_double),
+        ("fallback_count", ctypes.c_int32), ("reserved", ctypes.c_int32),
     ]
 
 
@@ -129,6 +131,7 @@ def lib():
         L.orc_rho_of_phi.restype = ctypes.c_double
         L.orc_rho_of_phi.argtypes = [ctypes.POINTER(_CParams), ctypes.c_double]
         L.orc_poisson_iterations.argtypes = [vp, ctypes.c_int]
+        L.orc_fixed_point_residual.argtypes = [vp, ctypes.c_double]
         u64p = ctypes.POINTER(ctypes.c_uint64)
         L.orc_set_schedule.argtypes = [vp, u64p, ctypes.c_int]
         L.orc_set_z_walls.argtypes = [vp, ctypes.c_int]
@@ -234,6 +237,12 @@ class Oracle:
         d = PoissonDiag()
         lib().orc_poisson_diag_get(self.h, ctypes.byref(d))
         return d
+
+    def fixed_point_residual(self, gauge_target: float) -> PoissonDiag:
+        """The dual-mode check's kinetic fixed-point residual of the current
+        state against gauge_target (fills the fixed_point_* diagnostics)."""
+        self._check(lib().orc_fixed_point_residual(self.h, float(gauge_target)))
+        return self.diag()
 
     def poisson_iterations(self, k: int) -> None:
         lib().orc_poisson_iterations(self.h, k)

@@ -36,14 +36,15 @@ def test_struct_sizes_match_header(tmp_path):
     # compile a tiny C program against the header and compare sizeof()
     src = str(tmp_path / "sizes_probe.c")
     with open(src, "w") as fh:
-        fh.write('#include <stdio.h>\n#include "lbm_hip.h"\nint main(void){printf("%zu %zu %zu %zu\\n",'
+        fh.write('#include <stdio.h>\n#include "lbm_hip.h"\nint main(void){printf("%zu %zu %zu %zu %zu\\n",'
                  ' sizeof(lbm_params_t), sizeof(lbm_poisson_diag_t), sizeof(lbm_perf_t),'
-                 ' sizeof(lbm_adjoint_stats_t)); return 0;}\n')
+                 ' sizeof(lbm_adjoint_stats_t), sizeof(lbm_poisson_check_t)); return 0;}\n')
     exe = src[:-2]
     subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
     sizes = list(map(int, subprocess.run([exe], capture_output=True, text=True).stdout.split()))
     assert sizes == [ctypes.sizeof(hip.LbmParams), ctypes.sizeof(hip.PoissonStepDiagnostics),
-                     ctypes.sizeof(hip.PerformanceMetrics), ctypes.sizeof(hip.AdjointStats)]
+                     ctypes.sizeof(hip.PerformanceMetrics), ctypes.sizeof(hip.AdjointStats),
+                     ctypes.sizeof(hip.PoissonCheck)]
 
 
 def test_null_arguments_are_rejected_without_touching_the_device():

@@ -211,3 +211,59 @@ def test_adjoint_rejects_mixed_context_and_bad_schedule():
     sim = hip.InamuroHIP(P)
     with pytest.raises(Exception, match="multiples"):
         sim.adjoint(phi0, target, 2, iterations=[100, 150])
+
+
+def test_failed_adjoint_leaves_the_settings_as_they_were():
+    """A forward pass that fails on the host-side numeric check (a NaN in
+    phi0 makes rho, tau_h and then the pressure NaN; every kernel exits
+    normally) must not leave the adjoint's frozen counts, its timing switch
+    or its check observer installed: afterwards lbm_upload + lbm_step
+    converge as on a fresh context, with the caller's own schedule in force
+    and timing still on."""
+    P = params()
+    phi0, target, _ = case(P)
+    sim = hip.InamuroHIP(P)
+    sim.setPoisson(check_interval=10)
+    sim.set_timing(True)
+    sim.set_iteration_schedule([40])  # the caller's own frozen count for one step
+    checks = []
+    sim.set_check_observer(checks.append)
+    bad = phi0.copy()
+    bad[4, 5, 6] = np.nan
+    announce("adjoint with a NaN in phi0 (expected to fail on the host check)")
+    with pytest.raises(RuntimeError, match="NaN/Inf"):
+        sim.adjoint(bad, target, 3, checkpoint_interval=1, iterations=[10, 200, 200])
+    assert checks == []  # no records from inside lbm_adjoint
+    ref = hip.InamuroHIP(P)
+    ref.setPoisson(check_interval=10)
+    for s_ in (sim, ref):
+        s_.initFromCPU(fei=phi0, rho=s_.initial_rho(phi0), u=0.0, v=0.0, w=0.0, p=0.0)
+    ref.set_iteration_schedule([40])
+    assert sim.step(3) == ref.step(3)
+    assert sim.iteration_history()[0] == 40
+    assert sim.perf().collide_ms > 0.0  # timing still on
+    assert len(checks) == sum(sim.iteration_history()) // 10
+    a, b = sim.downloadFieldsToCPU(), ref.downloadFieldsToCPU()
+    for k in ("fei", "rho", "u", "v", "w", "p"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_set_poisson_validates_every_argument_first():
+    """A rejected lbm_set_poisson changes no setting: neither an invalid dual
+    mode after a valid interval and tolerance, nor a new interval that the
+    stored schedule's counts are not multiples of."""
+    P = params()
+    sim = hip.InamuroHIP(P)
+    sim.set_iteration_schedule([200, 300])
+    with pytest.raises(ValueError, match="dual_residual"):
+        sim._check(load().lbm_set_poisson(sim._h, 50, 1e-4, 0, 7))
+    with pytest.raises(ValueError, match="multiples of this check interval"):
+        sim.setPoisson(200, 1e-4)
+    ref = hip.InamuroHIP(P)
+    ref.set_iteration_schedule([200, 300])
+    phi0, target, _ = case(P)
+    for s_ in (sim, ref):
+        s_.initFromCPU(fei=phi0, rho=s_.initial_rho(phi0), u=0.0, v=0.0, w=0.0, p=0.0)
+    got = sim.step(3)
+    assert got[:2] == [200, 300]
+    assert got == ref.step(3)  # the third step converges with the default interval and tolerance

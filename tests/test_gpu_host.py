@@ -36,9 +36,40 @@ def test_runner_matches_oracle_with_gpu_tree_initialisation(tmp_path):
     o = po.Oracle(P)
     o.place_droplet(0.0, P.ly / 2.0, P.lz / 2.0 + P.DD, vz=-0.035, profile="sharp")
     o.place_droplet(0.0, P.ly / 2.0, P.lz / 2.0 - P.DD, vz=0.035, profile="sharp")
-    iters = o.step(3)
-    rows = list(csv.DictReader(open(diag)))
-    assert [int(row["iteration"]) for row in rows] == iters
+    iters, finals = [], []
+    for _ in range(3):
+        iters += o.step(1)
+        finals.append(o.diag())
+    # The Poisson diagnostics CSV: the reference's 20-column header
+    # (GPU/InamuroCUDA.cu:2621-2626) and one row per convergence check.
+    with open(diag) as fh:
+        header = fh.readline().strip().split(",")
+        lines = fh.read().splitlines()
+    assert header == ["step", "iteration", "pressure_l1_delta", "pressure_l1_norm", "relative_error",
+                      "fixed_point_h_l1", "fixed_point_h_scale", "fixed_point_h_relative",
+                      "fixed_point_p_l1", "fixed_point_p_scale", "fixed_point_p_relative",
+                      "fixed_point_relative", "fixed_point_evaluated", "pressure_converged",
+                      "fixed_point_converged", "dual_residual_enabled", "converged",
+                      "block_low_frequency_fraction", "block_size", "block_count"]
+    rows = [dict(zip(header, ln.split(","))) for ln in lines]
+    assert all(len(ln.split(",")) == 20 for ln in lines)
+    want = [(s + 1, k) for s, n in enumerate(iters) for k in range(100, n + 1, 100)]
+    assert [(int(r_["step"]), int(r_["iteration"])) for r_ in rows] == want
+    for r_ in rows:
+        last = int(r_["iteration"]) == iters[int(r_["step"]) - 1]
+        assert r_["converged"] == r_["pressure_converged"] == ("1" if last else "0")
+        assert (float(r_["relative_error"]) < 1e-3) == last
+        # no dual residual and no spatial diagnostics: empty optional columns
+        assert r_["fixed_point_h_l1"] == r_["fixed_point_relative"] == ""
+        assert r_["fixed_point_evaluated"] == r_["dual_residual_enabled"] == "0"
+        assert (r_["block_low_frequency_fraction"], r_["block_size"], r_["block_count"]) == ("", "0", "0")
+        assert abs(float(r_["relative_error"]) - float(r_["pressure_l1_delta"]) / float(r_["pressure_l1_norm"])) \
+            <= 1e-15 * float(r_["relative_error"])
+    ends = [r_ for r_ in rows if r_["converged"] == "1"]
+    for r_, d in zip(ends, finals):
+        for k, ref in (("pressure_l1_delta", d.pressure_l1_delta), ("pressure_l1_norm", d.pressure_l1_norm),
+                       ("relative_error", d.relative_error)):
+            assert abs(float(r_[k]) - ref) <= 1e-10 * abs(ref), (k, r_[k], ref)
     got = np.fromfile(dump, dtype=np.float64).reshape(6, P.lz, P.ly, P.lx)
     for i, k in enumerate(FIELDS):
         assert rel_linf(got[i], o.get(k)) <= 1e-12, k

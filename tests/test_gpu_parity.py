@@ -68,6 +68,15 @@ def test_smoke_case_matches_oracle_every_step():
     # the gauge target is the mean of a zero-mean field: compare on the scale of p
     pscale = float(np.max(np.abs(b["p"])))
     assert abs(gd.pressure_gauge_target - od.pressure_gauge_target) <= 1e-12 * pscale
+    # Source statistics of the reference GPU solver (GPU/InamuroCUDA.cu:1891-1920):
+    # mean|div|/3 to 1e-12; the signed means are rounding-sized differences of
+    # O(|div|) terms, compared on the scale of mean|div|.
+    assert od.source_abs_mean > 0
+    assert abs(gd.source_abs_mean - od.source_abs_mean) <= 1e-12 * od.source_abs_mean
+    assert abs(gd.source_mean - od.source_mean) <= 1e-12 * od.source_abs_mean
+    assert abs(gd.projected_source_mean) <= 1e-13 * od.source_abs_mean
+    assert abs(od.projected_source_mean) <= 1e-13 * od.source_abs_mean
+    assert gd.fallback_count == od.fallback_count == 0
 
 
 def fixture_cases():
@@ -146,12 +155,45 @@ def test_dual_residual_exit_matches_oracle():
         assert rel_linf(a[k], b[k]) <= TOL, k
 
 
+DUAL_CASE = dict(lx=16, ly=16, lz=16, DD=8.0, rho_L=1000.0, rho_G=1.0, mu_L=0.008, mu_G=8e-5,
+                 k_g=1e-4, fei_max=0.093, fei_min=0.016, fei_L=0.093, fei_G=0.016)
+
+
+@pytest.mark.parametrize("dual", [False, True])
+def test_dual_residual_decides_the_exit(dual):
+    """With a check after every iteration the kinetic residual is still above
+    the tolerance when the pressure change first falls below it: the oracle
+    stops steps 2 and 3 of this density-ratio-1000 droplet after 784 and 542
+    iterations on the pressure change alone and after 800 and 554 with the
+    dual exit.  The GPU takes the same counts in both modes, so a wrong
+    AND-exit (either measure alone) would show."""
+    P = po.Params(**DUAL_CASE)
+    o, sim = start(P, (8.0, 8.0, 8.0), check_interval=1, tol=1e-3, iter_cap=0, dual=dual)
+    want = [1, 800, 554] if dual else [1, 784, 542]
+    for step in range(3):
+        print(f"dual={dual} step {step + 1}", flush=True)
+        gi, oi = sim.step(1), o.step(1)
+        assert gi == oi == [want[step]], (step + 1, gi, oi)
+        gd = sim.getLastPoissonDiagnostics()
+        assert gd.converged == 1 and gd.pressure_converged == 1
+        assert gd.relative_error < 1e-3
+        if dual:
+            assert gd.fixed_point_evaluated == 1 and gd.fixed_point_converged == 1
+            assert gd.fixed_point_relative < 1e-3
+    a, b = sim.downloadFieldsToCPU(), o.fields()
+    for k in FIELDS:
+        assert rel_linf(a[k], b[k]) <= TOL, k
+
+
 def test_iteration_limit_stops_like_the_reference_fallback():
     P = po.Params(lx=24, ly=24, lz=24, DD=12.0, rho_L=1000.0, rho_G=1.0, mu_G=8e-5, k_g=1e-4,
                   fei_max=0.093, fei_min=0.016, fei_L=0.093, fei_G=0.016)
     o, sim = start(P, (12.0, 12.0, 12.0), iter_cap=300)
     assert sim.step(2) == o.step(2) == [100, 300]
-    assert sim.getLastPoissonDiagnostics().hit_max_iterations == 1
+    gd = sim.getLastPoissonDiagnostics()
+    assert gd.hit_max_iterations == 1
+    # GPU/InamuroCUDA.cu:2335-2338 records the stop as one fallback.
+    assert gd.fallback_count == o.diag().fallback_count == 1
     a, b = sim.downloadFieldsToCPU(), o.fields()
     for k in FIELDS:
         assert rel_linf(a[k], b[k]) <= TOL, k
