@@ -385,11 +385,24 @@ def cpu_baseline(hip, wl, workload, n_it_per_step):
         sample=(f"{workload} initial state ({lx}x{ly}x{lz}): one collision+stream+boundary+macro+correction pass "
                 f"({t_np:.2f} s) and {k} Poisson iterations ({t_it:.3f} s each) timed on one host core, "
                 f"extrapolated to {n_it_per_step:.0f} iterations per step (the GPU's measured mean)"),
-        host_cpu=platform.processor() or platform.machine(), nproc=os.cpu_count(),
+        host_cpu=host_cpu_model(), nproc=os.cpu_count(),
         # measured where the reference compiles (one core, same inputs): the
         # oracle takes 0.58-0.60x the compiled reference CPU solver's time
         reference_estimate=0.59 * cells / t_step / 1e6,
     )
+
+
+def host_cpu_model() -> str:
+    """The host CPU's model name from /proc/cpuinfo (the machine type where
+    that file is absent)."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def main(argv=None):

@@ -409,66 +409,81 @@ void put_str(std::ofstream& o, const std::string& s)
 
 } // namespace
 
-// writeResults, CPU/Inamuro.cpp:365-381.  The Tecplot file keeps the
-// reference's #!TDV101 point-packed layout (CPU/Inamuro.cpp:865-1013):
-// variables X Y Z u v w rho fei press, coordinates as 1-based ints, values
-// as float32; the debug CSV keeps CPU/Inamuro.cpp:1015-1048's columns.
+// writeTecplotBinary, CPU/Inamuro.cpp:865-1013: the #!TDV101 point-packed
+// layout, variables X Y Z u v w rho fei press, coordinates as 1-based int32,
+// values as float32, x fastest.
+void writeTecplotBinary(const std::string& output_dir, int timeStep, int lx, int ly, int lz,
+                        const std::vector<double>* f)
+{
+    std::ostringstream name;
+    name << output_dir << "3D" << std::setfill('0') << std::setw(9) << timeStep << ".plt";
+    std::ofstream o(name.str(), std::ios::binary);
+    if (!o) {
+        std::cerr << "cannot open " << name.str() << std::endl;
+        return;
+    }
+    o.write("#!TDV101", 8);
+    put_i32(o, 1);
+    put_str(o, "Inamuro");
+    const char* vars[9] = {"X", "Y", "Z", "u", "v", "w", "rho", "fei", "press"};
+    put_i32(o, 9);
+    for (const char* v : vars) put_str(o, v);
+    put_f32(o, 299.0f);  // zone marker
+    put_str(o, "ZONE 001");
+    for (int v : {-1, 0, 1, 0, 0}) put_i32(o, v);  // colour, type, point packing, location, faces
+    put_i32(o, lx);
+    put_i32(o, ly);
+    put_i32(o, lz);
+    put_i32(o, 0);
+    put_f32(o, 357.0f);  // end of header
+    put_f32(o, 299.0f);
+    for (int v : {3, 3, 3, 1, 1, 1, 1, 1, 1}) put_i32(o, v);  // int32 coordinates, float32 values
+    put_i32(o, 0);
+    put_i32(o, -1);
+    for (int z = 0; z < lz; ++z)
+        for (int y = 0; y < ly; ++y)
+            for (int x = 0; x < lx; ++x) {
+                const std::size_t i = (static_cast<std::size_t>(z) * ly + y) * lx + x;
+                put_i32(o, x + 1);
+                put_i32(o, y + 1);
+                put_i32(o, z + 1);
+                for (int k : {2, 3, 4, 1, 0, 5}) put_f32(o, static_cast<float>(f[k][i]));
+            }
+}
+
+// writeDebugOutput, CPU/Inamuro.cpp:1015-1048: x,y,z (1-based), then
+// u, v, w, rho, fei, p in scientific notation with 6 digits.
+void writeDebugFieldCSV(const std::string& output_dir, int timeStep, int lx, int ly, int lz,
+                        const std::vector<double>* f)
+{
+    std::ostringstream name;
+    name << output_dir << "debug_" << std::setfill('0') << std::setw(9) << timeStep << ".csv";
+    std::ofstream o(name.str());
+    if (!o) {
+        std::cerr << "cannot open " << name.str() << std::endl;
+        return;
+    }
+    o << "x,y,z,u,v,w,rho,fei,p\n" << std::scientific << std::setprecision(6);
+    for (int z = 0; z < lz; ++z)
+        for (int y = 0; y < ly; ++y)
+            for (int x = 0; x < lx; ++x) {
+                const std::size_t i = (static_cast<std::size_t>(z) * ly + y) * lx + x;
+                o << (x + 1) << "," << (y + 1) << "," << (z + 1) << "," << f[2][i] << "," << f[3][i] << ","
+                  << f[4][i] << "," << f[1][i] << "," << f[0][i] << "," << f[5][i] << "\n";
+            }
+}
+
+// writeResults, CPU/Inamuro.cpp:365-381.
 void Inamuro::writeResults(int timeStep)
 {
     if (!output_config.enable_tecplot && !output_config.enable_debug_field_csv) return;
     if (!output_config.output_dir.empty()) std::filesystem::create_directories(output_config.output_dir);
     std::vector<double> f[6];
     hip.downloadFields(f);
-    const int lx = params.lx, ly = params.ly, lz = params.lz;
-    if (output_config.enable_tecplot) {
-        std::ostringstream name;
-        name << output_config.output_dir << "3D" << std::setfill('0') << std::setw(9) << timeStep << ".plt";
-        std::ofstream o(name.str(), std::ios::binary);
-        if (!o) {
-            std::cerr << "cannot open " << name.str() << std::endl;
-        } else {
-            o.write("#!TDV101", 8);
-            put_i32(o, 1);
-            put_str(o, "Inamuro");
-            const char* vars[9] = {"X", "Y", "Z", "u", "v", "w", "rho", "fei", "press"};
-            put_i32(o, 9);
-            for (const char* v : vars) put_str(o, v);
-            put_f32(o, 299.0f);
-            put_str(o, "ZONE 001");
-            for (int v : {-1, 0, 1, 0, 0}) put_i32(o, v);
-            put_i32(o, lx);
-            put_i32(o, ly);
-            put_i32(o, lz);
-            put_i32(o, 0);
-            put_f32(o, 357.0f);
-            put_f32(o, 299.0f);
-            for (int v : {3, 3, 3, 1, 1, 1, 1, 1, 1}) put_i32(o, v);
-            put_i32(o, 0);
-            put_i32(o, -1);
-            for (int z = 0; z < lz; ++z)
-                for (int y = 0; y < ly; ++y)
-                    for (int x = 0; x < lx; ++x) {
-                        const std::size_t i = (static_cast<std::size_t>(z) * ly + y) * lx + x;
-                        put_i32(o, x + 1);
-                        put_i32(o, y + 1);
-                        put_i32(o, z + 1);
-                        for (int k : {2, 3, 4, 1, 0, 5}) put_f32(o, static_cast<float>(f[k][i]));
-                    }
-        }
-    }
-    if (output_config.enable_debug_field_csv) {
-        std::ostringstream name;
-        name << output_config.output_dir << "debug_" << std::setfill('0') << std::setw(9) << timeStep << ".csv";
-        std::ofstream o(name.str());
-        o << "x,y,z,u,v,w,rho,fei,p\n" << std::scientific << std::setprecision(6);
-        for (int z = 0; z < lz; ++z)
-            for (int y = 0; y < ly; ++y)
-                for (int x = 0; x < lx; ++x) {
-                    const std::size_t i = (static_cast<std::size_t>(z) * ly + y) * lx + x;
-                    o << (x + 1) << "," << (y + 1) << "," << (z + 1) << "," << f[2][i] << "," << f[3][i]
-                      << "," << f[4][i] << "," << f[1][i] << "," << f[0][i] << "," << f[5][i] << "\n";
-                }
-    }
+    if (output_config.enable_tecplot)
+        writeTecplotBinary(output_config.output_dir, timeStep, params.lx, params.ly, params.lz, f);
+    if (output_config.enable_debug_field_csv)
+        writeDebugFieldCSV(output_config.output_dir, timeStep, params.lx, params.ly, params.lz, f);
 }
 
 } // namespace lbm

@@ -74,6 +74,15 @@ private:
     std::unique_ptr<std::ofstream> pdiag_;  // Poisson diagnostics CSV (heap: stable for the callback)
 };
 
+// The reference's output files, written from host fields f[6] = fei, rho,
+// u, v, w, p in the canonical x-fastest layout: Tecplot binary
+// <dir>3D<step:09>.plt (CPU/Inamuro.cpp:865-1013) and debug CSV
+// <dir>debug_<step:09>.csv (CPU/Inamuro.cpp:1015-1048).
+void writeTecplotBinary(const std::string& output_dir, int timeStep, int lx, int ly, int lz,
+                        const std::vector<double>* f);
+void writeDebugFieldCSV(const std::string& output_dir, int timeStep, int lx, int ly, int lz,
+                        const std::vector<double>* f);
+
 class Inamuro : public LBMBase {
 public:
     struct Parameters {

@@ -6,12 +6,14 @@
 // reference file is edited.  One simulation per process, because the
 // reference keeps function-static state (CPU/Inamuro.cpp:150, 254, 627-628).
 //
-// usage: ref_probe PARAMS INIT|- STEPS OUT [DUMP_DISTS]
+// usage: ref_probe PARAMS INIT|- STEPS OUT [DUMP_DISTS [OUTPUT_DIR]]
 //   INIT: 6 float64 fields (fei, rho, u, v, w, p), each [z][y][x] x-fastest,
 //         written into the reference's macro arrays at z+1 after construction
 //         ("-" keeps the reference's own two-droplet initialisation).
 //   OUT:  int32 {magic, lx, ly, lz, steps, dump}; uint64 iterations[steps];
 //         double step_ms[steps]; 6 fields; optionally f, g, h as [q][z][y][x].
+//   OUTPUT_DIR: after the steps, the reference's own writeResults(STEPS)
+//         writes its Tecplot file and debug CSV there (CPU/Inamuro.cpp:365-381).
 #include "Inamuro.hpp"
 
 #include <chrono>
@@ -92,6 +94,7 @@ int main(int argc, char** argv)
     const int steps = std::atoi(argv[3]);
     const std::string out_path = argv[4];
     const bool dists = argc > 5 && std::atoi(argv[5]) != 0;
+    const std::string output_dir = argc > 6 ? argv[6] : "";
 
     std::ofstream devnull("/dev/null");
     std::streambuf* saved = std::cout.rdbuf(devnull.rdbuf());
@@ -116,6 +119,14 @@ int main(int argc, char** argv)
             const auto t1 = std::chrono::steady_clock::now();
             step_ms[s] = std::chrono::duration<double, std::milli>(t1 - t0).count();
             iters[s] = probe.getLastPressureSolveDiagnostics().iteration_count;
+        }
+        if (!output_dir.empty()) {
+            Inamuro::OutputConfig oc;
+            oc.enable_tecplot = true;
+            oc.enable_debug_field_csv = true;
+            oc.output_dir = output_dir;
+            probe.setOutputConfig(oc);
+            probe.writeResults(steps);
         }
         std::cout.rdbuf(saved);
         std::FILE* out = std::fopen(out_path.c_str(), "wb");

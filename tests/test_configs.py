@@ -104,3 +104,55 @@ def test_perturbed_collision_fields():
     r = np.sqrt((x - np.where(x < 32, 20.0, 44.0)) ** 2 + (y - 16.0) ** 2 + (z - 16.0) ** 2)
     assert r.min() > 6.0 - 3.0 and r.max() < 6.0 + 3.0
     assert np.std(r) > 0.1  # not a sphere
+
+
+CFG3 = po.Params(lx=96, ly=96, lz=96, DD=34.4, rho_L=1000.0, rho_G=1.0, fei_max=0.093, fei_min=0.016,
+                 fei_L=0.093, fei_G=0.016)
+
+
+def test_ellipsoid_with_equal_axes_is_the_sphere_profile():
+    """s = r/R, |grad s| = r/R^2, so d = (1 - s) s / |grad s| = R - r: the
+    configurations' sphere profile mean + (phi_L - phi_G)/2 tanh((R - r)/2)."""
+    L, R = 32, 9.5
+    phi = configs.ellipsoid_phi(L, (R, R, R), CFG3)
+    c = L / 2.0
+    z, y, x = np.meshgrid(*(np.arange(L, dtype=np.float64) - c for _ in range(3)), indexing="ij")
+    r = np.sqrt(x * x + y * y + z * z)
+    want = 0.5 * (CFG3.fei_L + CFG3.fei_G) + 0.5 * (CFG3.fei_L - CFG3.fei_G) * np.tanh((R - r) / 2.0)
+    assert np.max(np.abs(phi - want)) <= 1e-16  # a few ulps of phi ~ 0.09
+    # and it agrees with the oracle's tanh droplet (place_droplet, interface
+    # width 2, phi_G / phi_L as fei_min / fei_max) to rounding
+    P = po.Params(lx=L, ly=L, lz=L, DD=2 * R, fei_max=CFG3.fei_L, fei_min=CFG3.fei_G,
+                  fei_L=CFG3.fei_L, fei_G=CFG3.fei_G)
+    o = po.Oracle(P)
+    o.place_droplet(c, c, c, profile="tanh", width=2.0)
+    assert np.max(np.abs(phi - o.get("fei"))) <= 1e-16
+
+
+def test_ellipsoid_mean_on_surface_and_distance_along_axes():
+    """phi equals the mean phi exactly where s = 1 (grid points on the
+    semi-axes 20, 16, 16 of the 96^3 configuration); along each axis the
+    first-order distance is exact, d = a - |x|; phi is above the mean inside
+    and below outside."""
+    L, axes = 96, (20.0, 16.0, 16.0)
+    phi = configs.ellipsoid_phi(L, axes, CFG3)
+    c = L // 2
+    mean = 0.5 * (CFG3.fei_L + CFG3.fei_G)
+    half = 0.5 * (CFG3.fei_L - CFG3.fei_G)
+    for (z, y, x) in [(c, c, c + 20), (c, c, c - 20), (c, c + 16, c), (c, c - 16, c),
+                      (c + 16, c, c), (c - 16, c, c)]:
+        assert phi[z, y, x] == mean, (z, y, x)
+    off = np.arange(L) - c
+    for axis, a in ((2, 20.0), (1, 16.0), (0, 16.0)):
+        idx = [c, c, c]
+        vals = []
+        for k in range(L):
+            idx[axis] = k
+            vals.append(phi[tuple(idx)])
+        vals = np.array(vals)
+        nz = off != 0
+        want = mean + half * np.tanh((a - np.abs(off[nz])) / 2.0)
+        assert np.max(np.abs(vals[nz] - want)) <= 1e-16, axis
+    z, y, x = np.meshgrid(*(np.arange(L, dtype=np.float64) - c for _ in range(3)), indexing="ij")
+    s = np.sqrt((x / axes[0]) ** 2 + (y / axes[1]) ** 2 + (z / axes[2]) ** 2)
+    assert np.all(phi[s < 0.999] > mean) and np.all(phi[s > 1.001] < mean)

@@ -231,3 +231,24 @@ def test_mass_is_the_fp64_sum_of_phi():
     m = sim.mass()
     ref = float(np.sum(o.get("fei"), dtype=np.float64))
     assert abs(m - ref) <= 1e-13 * abs(ref)
+
+
+@pytest.mark.parametrize("profile", ["tanh", "sharp"])
+def test_library_droplet_placement_equals_the_oracle(profile):
+    """lbm_place_droplet (the initial state of the 256^3 and falling-droplet
+    benchmarks) against the oracle's place_droplet, itself bit-identical to
+    the reference's GPU-tree constructor: two overlapping droplets with
+    velocities on the background state, every field bit for bit."""
+    P = po.Params(lx=14, ly=12, lz=16, DD=9.0, rho_L=1000.0, rho_G=1.0, fei_max=0.093, fei_min=0.016,
+                  fei_L=0.093, fei_G=0.016)
+    drops = [((6.5, 5.25, 8.0), (0.01, -0.02, 0.03), 1.5), ((9.0, 6.0, 4.5), (-0.015, 0.0, 0.005), 2.0)]
+    o = po.Oracle(P)
+    sim = hip.InamuroHIP(P)
+    sim.reset_fields()
+    for (cx, cy, cz), (vx, vy, vz), wd in drops:
+        o.place_droplet(cx, cy, cz, vx, vy, vz, profile=profile, width=wd)
+        sim.place_droplet(cx, cy, cz, vx, vy, vz, profile=profile, width=wd)
+    a, b = sim.downloadFieldsToCPU(), o.fields()
+    for k in FIELDS:
+        assert np.array_equal(a[k].view(np.uint64), b[k].view(np.uint64)), k
+    assert np.any(a["u"] != 0.0) and np.any(a["fei"] > P.fei_min)
