@@ -164,8 +164,8 @@ int lbm_create(const lbm_params_t* params, int precision, void** out_ctx);
 /* ~InamuroCUDA (GPU/InamuroCUDA.cu:1463-1467). */
 int lbm_destroy(void* ctx);
 
-/* Message of the last failure on ctx (or of the last failed lbm_create when
- * ctx is NULL). */
+/* Message of the last failure on ctx (or, when ctx is NULL, of the last
+ * failed call made without a context: lbm_create, lbm_mass_noalloc). */
 const char* lbm_last_error(const void* ctx);
 
 /* initFromCPU (GPU/InamuroCUDA.cu:1628-1673): uploads the macro fields
@@ -279,6 +279,18 @@ int lbm_perf_reset(void* ctx);
 
 /* Device global sums in FP64: sum of phi (mass) and of |u| over the grid. */
 int lbm_mass(void* ctx, double* sum_phi);
+
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops): the FP64 sum of
+ * lbm_mass over the caller's device array d_field[0..n), written to the
+ * device double *d_sum, bit-identical to lbm_mass on the same values.  Two
+ * kernel launches on `stream` (a hipStream_t; NULL = the null stream), no
+ * allocation, memset, copy or synchronisation: *d_sum holds the sum once the
+ * stream has run them.  Its block sums use a library-owned device array of
+ * 1024 doubles (8 KiB per GPU, held until the process ends) shared by every
+ * call on that GPU: calls that overlap on one GPU, on any streams, race.
+ * 1 <= n <= 2^31 - 1; the message of a rejected call is lbm_last_error(NULL). */
+int lbm_mass_noalloc(const double* d_field, int64_t n, double* d_sum, void* stream);
 
 /* Discrete adjoint (no counterpart in the reference; FP64 contexts only).
  * On every return, successful or not, the iteration schedule of

@@ -30,6 +30,7 @@ UNITS = [
     ("kernels_mixed.hip", ["-ffp-contract=fast"]),
     ("kernels_adjoint.hip", ["-ffp-contract=off"]),
     ("capi.cpp", ["-x", "hip", "-ffp-contract=off"]),
+    ("noalloc.cpp", ["-x", "hip", "-ffp-contract=off"]),
 ]
 
 

@@ -311,6 +311,10 @@ Ctx* create(const lbm_params_t& P, int precision)
         }
         reset_state(*c);
         sync(*c);
+        // lbm_mass_noalloc's scratch address on this device, resolved here
+        // so that later calls there need no set-up.
+        hipError_t se = hipSuccess;
+        if (!lbm::noalloc_scratch(c->device, &se)) throw HipError{se, "hipGetSymbolAddress(scratch)"};
     } catch (...) {
         free_all(*c);
         delete c;
@@ -1394,6 +1398,10 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
 }
 
 } // namespace
+
+namespace lbm {
+void set_global_error(const std::string& msg) { g_create_error = msg; }
+} // namespace lbm
 
 extern "C" {
 

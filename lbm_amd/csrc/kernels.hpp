@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <string>
 
 #include "d3q15.hpp"
 
@@ -177,6 +178,14 @@ void stencil(hipStream_t s, const Grid& g, const double* pc, double* lfei, doubl
 constexpr int PC_SLOTS = 24;  // per-cell adjoint slots of collide -> stencil
 
 } // namespace adj
+
+// Message of the last failed call made without a context (lbm_create,
+// lbm_mass_noalloc), returned by lbm_last_error(NULL).
+void set_global_error(const std::string& msg);
+// Device address of lbm_mass_noalloc's scratch array on device `dev` (which
+// must be current), resolved with hipGetSymbolAddress once per device;
+// nullptr with *err set on failure (noalloc.cpp).
+double* noalloc_scratch(int dev, hipError_t* err);
 
 // Mixed precision: FP32 storage and collision arithmetic for f, g, h (as
 // deviations) and the macro fields; FP64 pressure, p_prev and every global sum.

@@ -45,6 +45,7 @@ EXPORTED = (
     "lbm_adjoint", "lbm_set_adjoint_memory", "lbm_adjoint_stats", "lbm_set_poisson_graph",
     "lbm_set_z_walls", "lbm_set_gravity", "lbm_set_adjoint_objective",
     "lbm_set_adjoint_state_precision", "lbm_set_progress", "lbm_set_check_observer",
+    "lbm_mass_noalloc",
 )
 
 # lbm_progress_fn: (user, phase, step, iterations)
@@ -170,6 +171,7 @@ def load_library():
     L.lbm_set_progress.argtypes = [vp, PROGRESS_FN, vp]
     L.lbm_adjoint_stats.argtypes = [vp, ctypes.POINTER(AdjointStats)]
     L.lbm_set_check_observer.argtypes = [vp, CHECK_FN, vp]
+    L.lbm_mass_noalloc.argtypes = [vp, ctypes.c_int64, vp, vp]
     _lib = L
     return L
 
