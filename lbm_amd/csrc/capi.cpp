@@ -72,7 +72,7 @@ struct Ctx {
     void* h[2] = {nullptr, nullptr};
     void *fei = nullptr, *rho = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
     void *ux = nullptr, *vy = nullptr, *wz = nullptr;
-    void* tau = nullptr;  // FP64: tau_h;  mixed: 1 - 1/tau_h
+    void* tau = nullptr;  // FP64: tau_h;  mixed: 2 - 1/tau_h = 4/(rho + 2)
     void* src = nullptr;  // FP64: S;      mixed: S/3
     // FP64 in both precisions (p[1] unused).
     double* p[2] = {nullptr, nullptr};
@@ -268,6 +268,8 @@ Ctx* create(const lbm_params_t& P, int precision)
             static_cast<float>(P.a), static_cast<float>(P.b), static_cast<float>(P.fei_L),
             static_cast<float>(P.fei_G), static_cast<float>(c->ph.tmp1),
             static_cast<float>(c->ph.fei_avg), static_cast<float>(1.5 * (P.taug - 0.5) * 2.0)};
+        c->phf.drho = static_cast<float>(P.rho_L - P.rho_G);
+        c->phf.rho_arg = static_cast<float>(M_PI / 2.0 / (P.fei_L - P.fei_G));
         HIPCHK(hipStreamCreate(&c->stream));
         const size_t n = c->n();
         const size_t fb = n * c->esz;

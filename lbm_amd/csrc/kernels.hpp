@@ -194,6 +194,8 @@ namespace mx {
 struct PhysF {
     float rho_L, rho_G, mu_G, tauf_inv, taug_inv, k_f, k_g, T, a, b, fei_L, fei_G;
     float tmp1, fei_avg, visc;  // visc = 1.5 * (taug - 0.5) * 2.0
+    float drho = 0.f;            // rho_L - rho_G
+    float rho_arg = 0.f;         // (pi/2) / (fei_L - fei_G)
     float gz = 0.f;             // extension: gravitational acceleration along z
 };
 

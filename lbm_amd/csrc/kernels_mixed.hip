@@ -287,7 +287,8 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
 }
 
 // getMacro (CPU/Inamuro.cpp:294-346) on the pulled deviations; re-bases f'_0
-// on the new phi; writes omega = 1 - 1/tau_h and S/3 for the Poisson solve.
+// on the new phi; writes omega = 2 - 1/tau_h (the relaxation factor
+// 1 - 1/tau_h plus one) and S/3 for the Poisson solve.
 __device__ __forceinline__ double macro_cell(
     int64_t c, const Grid& g, const PhysF& ph, float* __restrict__ fpost, const float* __restrict__ gpost,
     const float* __restrict__ ux, const float* __restrict__ vy, const float* __restrict__ wz,
@@ -311,8 +312,13 @@ __device__ __forceinline__ double macro_cell(
     } else if (phi >= ph.fei_L) {
         density = ph.rho_L;
     } else {
-        const float arg = (phi - ph.fei_avg) / (ph.fei_L - ph.fei_G) * 3.14159265358979f;
-        density = (ph.rho_L - ph.rho_G) * 0.5f * (sinf(arg) + 1.0f) + ph.rho_G;
+        // (rho_L - rho_G)/2 (sin(pi (phi - mean)/(phi_L - phi_G)) + 1) + rho_G
+        // (CPU/Inamuro.cpp:313-327) rewritten with 1 + sin(x) = 2 sin^2(x/2 + pi/4)
+        // as rho_G + (rho_L - rho_G) sin^2(pi/2 (phi - phi_G)/(phi_L - phi_G)).
+        // Equal in exact arithmetic; in FP32 sin(x) + 1 cancels on the gas side
+        // of the interface (sin ~ -1), which cost rho there up to 3e-5 relative.
+        const float sn = sinf((phi - ph.fei_G) * ph.rho_arg);
+        density = ph.rho_G + ph.drho * (sn * sn);
     }
     float uu = 0.f, vv = 0.f, ww = 0.f;
 #pragma unroll
@@ -327,8 +333,12 @@ __device__ __forceinline__ double macro_cell(
     u[c] = uu;
     v[c] = vv;
     w[c] = ww;
-    const float tauh = 1.0f / density + 0.5f;
-    omega[c] = 1.0f - 1.0f / tauh;
+    // 1 + (1 - 1/tau_h) = 2 - 1/tau_h with tau_h = 1/rho + 1/2, i.e.
+    // 4/(rho + 2): about 0.004 in the liquid, kept to FP32's relative
+    // precision (1 - 1/tau_h itself is -0.996 there, and rounding it to FP32
+    // would shift the small part that sets the pressure's diffusivity by up
+    // to 1.5e-5 relative).
+    omega[c] = 4.0f / (density + 2.0f);
     const float m = static_cast<float>(st->div_mean);
     const float sc = ((ux[c] - m) + vy[c]) + wz[c];
     src3[c] = sc * (1.0f / 3.0f);
@@ -430,8 +440,9 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_poisson_push(
     const int zm = wrap(z, -1, g.lz), zp = wrap(z, 1, g.lz);
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        // h* - E_q b = (h - E_q p)(1 - 1/tau_h) - E_q S/3 + E_q (p - b)
-        const float o = dev[q] * om - EWF[q] * s3 + static_cast<float>(EW[q] * rb);
+        // h* - E_q b = (h - E_q p)(1 - 1/tau_h) - E_q S/3 + E_q (p - b), with
+        // d (1 - 1/tau_h) = d a - d formed in one rounding (a = 2 - 1/tau_h)
+        const float o = fmaf(dev[q], om, -dev[q]) - EWF[q] * s3 + static_cast<float>(EW[q] * rb);
         const int dx = CX[q] == 1 ? xp : (CX[q] == -1 ? xm : x);
         const int dy = CY[q] == 1 ? yp : (CY[q] == -1 ? ym : y);
         const int dz = CZ[q] == 1 ? zp : (CZ[q] == -1 ? zm : z);
@@ -505,7 +516,7 @@ __global__ __launch_bounds__(BLOCK) void k_fixed_point(
             const double Hq = EW[q] * static_cast<double>(pbase[lin(ssx, ssy, ssz, g)]) +
                               static_cast<double>(hin[qq * g.n + s]) - EW[q] * shift;
             const double ps = p[s];
-            im[q] = EW[q] * ps + ((Hq - EW[q] * ps) * static_cast<double>(omega[s]) -
+            im[q] = EW[q] * ps + ((Hq - EW[q] * ps) * (static_cast<double>(omega[s]) - 1.0) -
                                   EW[q] * static_cast<double>(src3[s]));
         }
         slip(im, x, g.lx);
