@@ -220,6 +220,17 @@ int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance,
  * either way; only launch overhead changes.  On by default here. */
 int lbm_set_poisson_graph(void* ctx, int enabled);
 
+/* Launch shape of the Poisson iteration kernel (no counterpart in the
+ * reference): -1, the default, times the candidates on the context's own
+ * buffers before the first solve and keeps the fastest for this grid; 0..2
+ * fixes one (FP64: 256 threads with non-temporal loads of h, 256 plain, 768
+ * plain; mixed: 1024 threads with non-temporal loads, 512 with, 512 plain).
+ * Every choice gives bit-identical results; only the speed differs with the
+ * grid's shape.  lbm_get_poisson_launch reports the one in use (-1 before the
+ * first solve when choosing automatically). */
+int lbm_set_poisson_launch(void* ctx, int config);
+int lbm_get_poisson_launch(const void* ctx, int* config);
+
 /* No counterpart in the reference (extensions for the falling-droplet
  * workload), both off by default:
  * lbm_set_z_walls: no-slip walls at z = 0 and z = lz-1 -- halfway bounce-back

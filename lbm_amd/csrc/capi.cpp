@@ -119,10 +119,12 @@ struct Ctx {
         int hcur_after, pb_after;
     };
     bool use_graphs = true;
-    // FP64 Poisson loads with a non-temporal hint when the two h buffers
-    // exceed the 256 MiB last-level cache (measured: 1.4% faster at 256^3,
-    // 4% slower at 96^3 where the pair stays cache-resident).
-    bool nt_loads = false;
+    // Launch shape of the Poisson push kernel (block size and load hint, one
+    // of the precision's PUSH_CONFIGS; every one gives the same results).
+    // push_request < 0: time the candidates at the first solve and keep the
+    // fastest (push_cfg < 0 until then).
+    int push_request = -1;
+    int push_cfg = -1;
     std::vector<SegGraph> graphs;
 
     bool timing = false;
@@ -255,7 +257,6 @@ Ctx* create(const lbm_params_t& P, int precision)
         c->esz = c->mixed ? sizeof(float) : sizeof(double);
         HIPCHK(hipGetDevice(&c->device));
         c->g = Grid{P.lx, P.ly, P.lz, static_cast<int64_t>(P.lx) * P.ly * P.lz};
-        c->nt_loads = 2.0 * lbm::Q * sizeof(double) * static_cast<double>(c->g.n) > 256.0 * 1024 * 1024;
         c->ph = Phys{P.rho_L, P.rho_G, P.mu_L, P.mu_G, P.tauf, P.taug, P.k_f, P.k_g, P.T, P.a, P.b,
                      P.fei_L, P.fei_G,
                      (P.mu_L - P.mu_G) / (P.rho_L - P.rho_G),
@@ -394,22 +395,61 @@ void step_fg(Ctx& c)
     record(c, 2);
 }
 
-// One Poisson iteration; updates the buffer bookkeeping.
-void poisson_iteration(Ctx& c, int mode)
+// The push kernel's launch on the current buffers, in config `cfg`.
+void launch_push(Ctx& c, int mode, int cfg)
 {
     const Grid& g = c.g;
     if (!c.mixed) {
         lbm::f64::poisson_iter(c.stream, mode, g, D<double>(c.h[c.hcur]), D<double>(c.h[1 - c.hcur]),
-                               c.p[c.pc], D<double>(c.tau), D<double>(c.src), c.dst, c.nt_loads);
+                               c.p[c.pc], D<double>(c.tau), D<double>(c.src), c.dst, cfg);
     } else {
         // Decodes with base pb.  The first iteration of a segment (equilibrium,
         // explicit or gauged entry) writes new bases into the other buffer;
         // streamed iterations encode against base pb itself.
         lbm::mx::poisson_iter(c.stream, mode, g, D<float>(c.h[c.hcur]), D<float>(c.h[1 - c.hcur]),
                               c.base[c.pb], c.p[c.pc], c.base[1 - c.pb], D<float>(c.tau),
-                              D<float>(c.src), c.dst);
-        if (mode != lbm::H_STREAMED) c.pb = 1 - c.pb;
+                              D<float>(c.src), c.dst, cfg);
     }
+}
+
+// Picks the push kernel's launch shape for this grid.  With push_request < 0
+// each candidate runs the steady-state (streamed) iteration from h[hcur] into
+// h[1 - hcur] a few times under HIP events, and the fastest is kept.  This
+// writes nothing the next real iteration does not overwrite in full (the push
+// writes every population of every cell of the output buffer; streamed
+// iterations write no bases), so the state is unchanged.  Called outside graph
+// capture, before the first solve's first segment.
+void choose_push_config(Ctx& c)
+{
+    if (c.push_cfg >= 0) return;
+    const int ncfg = c.mixed ? lbm::mx::PUSH_CONFIGS : lbm::f64::PUSH_CONFIGS;
+    if (c.push_request >= 0) {
+        c.push_cfg = c.push_request;
+        return;
+    }
+    // Three interleaved rounds, each candidate's best round counts: run-to-run
+    // spread is a few percent, about what separates the candidates.
+    constexpr int ROUNDS = 3, WARM = 2, TIMED = 5;
+    std::vector<float> best(static_cast<size_t>(ncfg), std::numeric_limits<float>::max());
+    for (int round = 0; round < ROUNDS; ++round) {
+        for (int cfg = 0; cfg < ncfg; ++cfg) {
+            for (int k = 0; k < WARM; ++k) launch_push(c, lbm::H_STREAMED, cfg);
+            HIPCHK(hipEventRecord(c.ev[3], c.stream));
+            for (int k = 0; k < TIMED; ++k) launch_push(c, lbm::H_STREAMED, cfg);
+            HIPCHK(hipEventRecord(c.ev[9], c.stream));
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipEventSynchronize(c.ev[9]));
+            best[static_cast<size_t>(cfg)] = std::min(best[static_cast<size_t>(cfg)], elapsed(c.ev[3], c.ev[9]));
+        }
+    }
+    c.push_cfg = static_cast<int>(std::min_element(best.begin(), best.end()) - best.begin());
+}
+
+// One Poisson iteration; updates the buffer bookkeeping.
+void poisson_iteration(Ctx& c, int mode)
+{
+    launch_push(c, mode, c.push_cfg);
+    if (c.mixed && mode != lbm::H_STREAMED) c.pb = 1 - c.pb;
     c.hcur = 1 - c.hcur;
 }
 
@@ -522,6 +562,7 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
 
     int hmode = c.hs == HState::Equilibrium ? lbm::H_EQUILIBRIUM : lbm::H_EXPLICIT;
     const uint64_t forced = c.sched_pos < c.schedule.size() ? c.schedule[c.sched_pos] : 0;
+    choose_push_config(c);
     uint64_t iter = 0;
     bool first_segment = true;
     for (;;) {
@@ -902,7 +943,7 @@ void adj_forward_block(Ctx& c, AdjWork& a, const double* p_start, double* start,
     for (uint64_t k = a0 + 1; k <= a1; ++k) {
         double* out = a.slot(c, k - a0);
         lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, plast, D<double>(c.tau), D<double>(c.src), c.dst,
-                               true);
+                               0);
         if (k % ci == 0) {
             double* pk = a.pstore + ((k - a0) / ci) * n;
             lbm::f64::gauge(c.stream, c.g, c.ph, out, pk, c.p_prev, c.partials, c.dst);
@@ -942,7 +983,7 @@ void adj_forward_snaps(Ctx& c, AdjWork& a, const double* p_start, const double* 
         const SnapView v = snap_view(c, mark ? marks[m].second : a.slot(c, 1));
         double* out = mark ? v.h : a.slot(c, 1 + (k & 1));
         lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, plast, D<double>(c.tau), D<double>(c.src), c.dst,
-                               true);
+                               0);
         if (k % ci == 0) {
             double* pk = mark ? v.p : a.pstore + ((k / ci) & 1) * n;
             lbm::f64::gauge(c.stream, c.g, c.ph, out, pk, c.p_prev, c.partials, c.dst);
@@ -1609,6 +1650,24 @@ int lbm_set_gravity(void* ctx, double gz)
         c.ph.gz = gz;
         c.phf.gz = static_cast<float>(gz);
     });
+}
+
+int lbm_set_poisson_launch(void* ctx, int config)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        if (config < -1 || config >= lbm::f64::PUSH_CONFIGS)
+            throw ApiError{LBM_ERR_ARG, "Poisson launch config must be -1 (automatic) or 0..2"};
+        c.push_request = config;
+        c.push_cfg = -1;
+        clear_graphs(c);
+    });
+}
+
+int lbm_get_poisson_launch(const void* ctx, int* config)
+{
+    if (!ctx || !config) return LBM_ERR_ARG;
+    *config = static_cast<const Ctx*>(ctx)->push_cfg;
+    return LBM_OK;
 }
 
 int lbm_set_poisson_graph(void* ctx, int enabled)

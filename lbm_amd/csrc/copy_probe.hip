@@ -64,12 +64,13 @@ __constant__ int D3Q15_C[15][3] = {{0, 0, 0},  {1, 0, 0},  {0, 1, 0},   {0, 0, 1
                                    {0, -1, 0}, {0, 0, -1}, {1, 1, 1},   {-1, 1, 1},  {1, -1, 1},
                                    {1, 1, -1}, {-1, -1, -1}, {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}};
 
-__global__ __launch_bounds__(256) void copy_d3q15(const double* __restrict__ in, double* __restrict__ out,
+template <typename T>
+__global__ __launch_bounds__(256) void copy_d3q15(const T* __restrict__ in, T* __restrict__ out,
                                                   long long lx, long long ly, long long n)
 {
     const long long i = static_cast<long long>(blockIdx.x) * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    double v[15];
+    T v[15];
 #pragma unroll
     for (int q = 0; q < 15; ++q) v[q] = in[q * n + i];
 #pragma unroll
@@ -120,13 +121,25 @@ int ceiling()
               [&] { copy_soa15<0><<<gd, 256>>>(h0, h1, nd); })) return 1;
     if (timed("15 SoA arrays in, 15 out shifted by +-1 cell", 2.0 * 15 * nd * sizeof(double),
               [&] { copy_soa15<1><<<gd, 256>>>(h0, h1, nd); })) return 1;
-    const long long shapes[4][3] = {{256, 256, 256}, {256, 256, 216}, {192, 192, 384}, {384, 192, 192}};
+    const long long shapes[5][3] = {{256, 256, 256}, {256, 256, 216}, {192, 192, 384}, {384, 192, 192},
+                                    {256, 128, 128}};
     for (const auto& sh : shapes) {
         const long long n = sh[0] * sh[1] * sh[2];
         char name[96];
         std::snprintf(name, sizeof(name), "D3Q15 push pattern %lldx%lldx%lld", sh[0], sh[1], sh[2]);
         if (timed(name, 2.0 * 15 * n * sizeof(double), [&] {
-                copy_d3q15<<<static_cast<unsigned>((n + 255) / 256), 256>>>(h0, h1, sh[0], sh[1], n);
+                copy_d3q15<double><<<static_cast<unsigned>((n + 255) / 256), 256>>>(h0, h1, sh[0], sh[1], n);
+            }))
+            return 1;
+    }
+    // The same pattern on 4-byte values (the mixed-precision iteration's h).
+    for (const auto& sh : {shapes[0], shapes[2], shapes[4]}) {
+        const long long n = sh[0] * sh[1] * sh[2];
+        char name[96];
+        std::snprintf(name, sizeof(name), "D3Q15 push pattern FP32 %lldx%lldx%lld", sh[0], sh[1], sh[2]);
+        if (timed(name, 2.0 * 15 * n * sizeof(float), [&] {
+                copy_d3q15<float><<<static_cast<unsigned>((n + 255) / 256), 256>>>(
+                    reinterpret_cast<const float*>(h0), reinterpret_cast<float*>(h1), sh[0], sh[1], n);
             }))
             return 1;
     }

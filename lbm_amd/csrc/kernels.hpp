@@ -97,9 +97,12 @@ void macro(hipStream_t s, int pull, const Grid& g, const Phys& ph, const double*
            DevState* st, double* fei, double* rho, double* u, double* v, double* w,
            double* tau, double* src, double* partials);
 void projected_finalize(hipStream_t s, const double* parts, int nparts, DevState* st);
+// One Poisson iteration; config selects the launch shape (block size and
+// load hint) among PUSH_CONFIGS, with results identical for all of them.
+constexpr int PUSH_CONFIGS = 3;
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, double* hout,
                   const double* p, const double* tau, const double* src, const DevState* st,
-                  bool nontemporal);
+                  int config);
 void check(hipStream_t s, const Grid& g, const Phys& ph, double* p, double* p_prev,
            const double* hin, const double* tau, const double* src, int dual, double tol,
            double* partials, DevState* st);
@@ -206,9 +209,10 @@ void collide(hipStream_t s, int mode, const Grid& g, const PhysF& ph, const floa
 void macro(hipStream_t s, const Grid& g, const PhysF& ph, float* fpost, const float* gpost,
            const float* ux, const float* vy, const float* wz, DevState* st, float* fei,
            float* rho, float* u, float* v, float* w, float* omega, float* src3, double* partials);
+constexpr int PUSH_CONFIGS = 3;
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const float* hin, float* hout,
                   const float* bbase, const double* pown, float* bout, const float* omega,
-                  const float* src3, const DevState* st);
+                  const float* src3, const DevState* st, int config);
 void getp_sum(hipStream_t s, const Grid& g, const float* hin, const float* pbase, double* p,
               double* parts);
 void fixed_point(hipStream_t s, const Grid& g, const float* hin, const float* pbase,

@@ -391,15 +391,15 @@ __device__ __forceinline__ void load_streamed(const double* __restrict__ hin, in
 // slip rule and the pressure sum are evaluated moves, so all loads are the
 // cell's own (aligned, coalesced) and p is written only at checks.
 // NT: the streamed h is loaded with a non-temporal hint (it is dead after
-// this read), for grids whose two h buffers do not fit the last-level cache
-// and for the adjoint's recomputation.
-template <int MODE, int NT>
-__global__ __launch_bounds__(BLOCK) void k_poisson_push(
+// this read).  PB: threads per block.  Which (PB, NT) pair is fastest depends
+// on the grid's shape; the caller picks one of PUSH_CONFIGS.
+template <int MODE, int NT, int PB>
+__global__ __launch_bounds__(PB) void k_poisson_push(
     Grid g, const double* __restrict__ hin, double* __restrict__ hout,
     const double* __restrict__ pbuf, const double* __restrict__ tau,
     const double* __restrict__ src, const DevState* __restrict__ st)
 {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * PB + threadIdx.x;
     if (c >= g.n) return;
     int x, y, z;
     cell_xyz(c, g, x, y, z);
@@ -786,15 +786,21 @@ void projected_finalize(hipStream_t s, const double* parts, int nparts, DevState
 
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, double* hout,
                   const double* p, const double* tau, const double* src, const DevState* st,
-                  bool nontemporal)
+                  int config)
 {
-    const int nb = grid_blocks(g.n);
+    // PUSH_CONFIGS: 0 = 256 threads with non-temporal loads, 1 = 256 plain,
+    // 2 = 768 plain.
+#define LBM_PUSH_AT(M, NT, PB)                                                                     \
+    k_poisson_push<M, NT, PB><<<static_cast<int>((g.n + PB - 1) / PB), PB, 0, s>>>(g, hin, hout, p, \
+                                                                                  tau, src, st)
 #define LBM_PUSH(M)                                                                          \
     do {                                                                                     \
-        if (nontemporal)                                                                     \
-            k_poisson_push<M, 1><<<nb, BLOCK, 0, s>>>(g, hin, hout, p, tau, src, st);        \
+        if (config == 1)                                                                     \
+            LBM_PUSH_AT(M, 0, 256);                                                          \
+        else if (config == 2)                                                                \
+            LBM_PUSH_AT(M, 0, 768);                                                          \
         else                                                                                 \
-            k_poisson_push<M, 0><<<nb, BLOCK, 0, s>>>(g, hin, hout, p, tau, src, st);        \
+            LBM_PUSH_AT(M, 1, 256);                                                          \
     } while (0)
     switch (mode) {
     case H_EQUILIBRIUM: LBM_PUSH(H_EQUILIBRIUM); break;
@@ -803,6 +809,7 @@ void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, dou
     default: LBM_PUSH(H_STREAMED); break;
     }
 #undef LBM_PUSH
+#undef LBM_PUSH_AT
 }
 
 void check(hipStream_t s, const Grid& g, const Phys& ph, double* p, double* p_prev,

@@ -26,11 +26,6 @@
 namespace lbm {
 namespace mx {
 
-// Threads per block of the Poisson push kernel: two 256-cell x rows share a
-// block's cache lines of neighbour bases (measured 1% faster than 256 at 256^3;
-// 128 and 1024 are slower).
-constexpr int PUSH_BLOCK = 512;
-
 namespace {
 
 constexpr float EWF[Q] = {2.0f / 9.0f,
@@ -363,6 +358,7 @@ __global__ __launch_bounds__(BLOCK) void k_macro(
 
 // Reconstructs the post-boundary FP64 pressure populations of cell c from the
 // streamed deviations and the FP64 pressures of the cells they came from.
+template <int NT = 0>
 __device__ __forceinline__ void decode_h(const float* __restrict__ hin, const float* __restrict__ pbase,
                                          int64_t c, int x, int y, int z, const Grid& g, double (&H)[Q])
 {
@@ -376,21 +372,24 @@ __device__ __forceinline__ void decode_h(const float* __restrict__ hin, const fl
         const int sz = CZ[q] == 1 ? zm : (CZ[q] == -1 ? zp : z);
         // a value bounced at a z wall was encoded by this cell, with its base
         const int64_t src = z_bounced_in(q, z, g) ? c : lin(sx, sy, sz, g);
-        H[q] = EW[q] * static_cast<double>(pbase[src]) + static_cast<double>(hin[q * g.n + c]);
+        const float hq = NT ? __builtin_nontemporal_load(&hin[q * g.n + c]) : hin[q * g.n + c];
+        H[q] = EW[q] * static_cast<double>(pbase[src]) + static_cast<double>(hq);
     }
     slip(H, x, g.lx);
 }
 
 // One Poisson iteration in push form (see kernels_f64.hip for the FP64 twin).
 // Every mode writes this cell's FP32 base b = float(p) for the next decode.
-template <int MODE>
-__global__ __launch_bounds__(PUSH_BLOCK) void k_poisson_push(
+// NT: the streamed h is loaded with a non-temporal hint; PB: threads per
+// block (the caller picks one of PUSH_CONFIGS).
+template <int MODE, int NT, int PB>
+__global__ __launch_bounds__(PB) void k_poisson_push(
     Grid g, const float* __restrict__ hin, float* __restrict__ hout,
     const float* __restrict__ bbase, const double* __restrict__ pown, float* __restrict__ bout,
     const float* __restrict__ omega, const float* __restrict__ src3,
     const DevState* __restrict__ st)
 {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * PUSH_BLOCK + threadIdx.x;
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * PB + threadIdx.x;
     if (c >= g.n) return;
     int x, y, z;
     cell_xyz(c, g, x, y, z);
@@ -407,7 +406,7 @@ __global__ __launch_bounds__(PUSH_BLOCK) void k_poisson_push(
             for (int q = 0; q < Q; ++q) H[q] = static_cast<double>(hin[q * g.n + c]);
             pn = pown[c];
         } else {
-            decode_h(hin, bbase, c, x, y, z, g, H);
+            decode_h<NT>(hin, bbase, c, x, y, z, g, H);
             if (MODE == H_GAUGED) {
                 const double shift = st->shift;
 #pragma unroll
@@ -669,23 +668,30 @@ void macro(hipStream_t s, const Grid& g, const PhysF& ph, float* fpost, const fl
 
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const float* hin, float* hout,
                   const float* bbase, const double* pown, float* bout, const float* omega,
-                  const float* src3, const DevState* st)
+                  const float* src3, const DevState* st, int config)
 {
-    const int push_blocks = static_cast<int>((g.n + PUSH_BLOCK - 1) / PUSH_BLOCK);
+    // PUSH_CONFIGS: 0 = 1024 threads with non-temporal loads, 1 = 512 with
+    // non-temporal loads, 2 = 512 plain.
+#define LBM_MX_PUSH_AT(M, NT, PB)                                                                    \
+    k_poisson_push<M, NT, PB><<<static_cast<int>((g.n + PB - 1) / PB), PB, 0, s>>>(                  \
+        g, hin, hout, bbase, pown, bout, omega, src3, st)
+#define LBM_MX_PUSH(M)                                                                         \
+    do {                                                                                       \
+        if (config == 1)                                                                       \
+            LBM_MX_PUSH_AT(M, 1, 512);                                                         \
+        else if (config == 2)                                                                  \
+            LBM_MX_PUSH_AT(M, 0, 512);                                                         \
+        else                                                                                   \
+            LBM_MX_PUSH_AT(M, 1, 1024);                                                        \
+    } while (0)
     switch (mode) {
-    case H_EQUILIBRIUM:
-        k_poisson_push<H_EQUILIBRIUM><<<push_blocks, PUSH_BLOCK, 0, s>>>(g, hin, hout, bbase, pown, bout, omega, src3, st);
-        break;
-    case H_GAUGED:
-        k_poisson_push<H_GAUGED><<<push_blocks, PUSH_BLOCK, 0, s>>>(g, hin, hout, bbase, pown, bout, omega, src3, st);
-        break;
-    case H_EXPLICIT:
-        k_poisson_push<H_EXPLICIT><<<push_blocks, PUSH_BLOCK, 0, s>>>(g, hin, hout, bbase, pown, bout, omega, src3, st);
-        break;
-    default:
-        k_poisson_push<H_STREAMED><<<push_blocks, PUSH_BLOCK, 0, s>>>(g, hin, hout, bbase, pown, bout, omega, src3, st);
-        break;
+    case H_EQUILIBRIUM: LBM_MX_PUSH(H_EQUILIBRIUM); break;
+    case H_GAUGED: LBM_MX_PUSH(H_GAUGED); break;
+    case H_EXPLICIT: LBM_MX_PUSH(H_EXPLICIT); break;
+    default: LBM_MX_PUSH(H_STREAMED); break;
     }
+#undef LBM_MX_PUSH
+#undef LBM_MX_PUSH_AT
 }
 
 void getp_sum(hipStream_t s, const Grid& g, const float* hin, const float* pbase, double* p,

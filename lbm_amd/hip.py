@@ -45,7 +45,7 @@ EXPORTED = (
     "lbm_adjoint", "lbm_set_adjoint_memory", "lbm_adjoint_stats", "lbm_set_poisson_graph",
     "lbm_set_z_walls", "lbm_set_gravity", "lbm_set_adjoint_objective",
     "lbm_set_adjoint_state_precision", "lbm_set_progress", "lbm_set_check_observer",
-    "lbm_mass_noalloc",
+    "lbm_mass_noalloc", "lbm_set_poisson_launch", "lbm_get_poisson_launch",
 )
 
 # lbm_progress_fn: (user, phase, step, iterations)
@@ -172,6 +172,8 @@ def load_library():
     L.lbm_adjoint_stats.argtypes = [vp, ctypes.POINTER(AdjointStats)]
     L.lbm_set_check_observer.argtypes = [vp, CHECK_FN, vp]
     L.lbm_mass_noalloc.argtypes = [vp, ctypes.c_int64, vp, vp]
+    L.lbm_set_poisson_launch.argtypes = [vp, ctypes.c_int]
+    L.lbm_get_poisson_launch.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
     _lib = L
     return L
 
@@ -326,6 +328,16 @@ class InamuroHIP:
     def setUsePoissonDualResidual(self, enabled: bool):
         ci, tol, lim, _ = self._cfg()
         self.setPoisson(ci, tol, lim, enabled)
+
+    def set_poisson_launch(self, config: int = -1):
+        """Launch shape of the Poisson kernel: -1 chooses by timing at the
+        first solve (default), 0..2 fixes one; results are identical."""
+        self._check(load_library().lbm_set_poisson_launch(self._h, int(config)))
+
+    def poisson_launch(self) -> int:
+        out = ctypes.c_int()
+        self._check(load_library().lbm_get_poisson_launch(self._h, ctypes.byref(out)))
+        return out.value
 
     def set_z_walls(self, enabled: bool = True):
         """No-slip walls at z = 0 and lz-1 (extension; off by default)."""

@@ -41,3 +41,40 @@ def test_graph_replay_is_bitwise_eager(precision, dual):
     assert max(it_g) > 100  # several segments per solve
     for k in a:
         assert np.array_equal(a[k], b[k]), k
+
+
+@pytest.mark.parametrize("precision", [hip.FP64, hip.MIXED], ids=["fp64", "mixed"])
+def test_every_poisson_launch_shape_gives_the_same_bits(precision):
+    """The Poisson kernel's launch shapes (block size and load hint, chosen
+    per grid by timing them at the first solve) compute every cell the same
+    way: fields, distributions and iteration counts are bitwise equal for
+    each fixed shape and for the automatic choice, on a grid whose rows are
+    not a multiple of any block size (lx = 20) and that enters the solve in
+    each mode."""
+    P = po.Params(lx=20, ly=16, lz=24, DD=8.0, rho_L=100.0)
+    o = po.Oracle(P)
+    o.place_droplet(P.lx / 2, P.ly / 2, P.lz / 2, vz=0.01, profile="tanh", width=2.0)
+    start = o.fields()
+    o.close()
+    outs = {}
+    for cfg in (-1, 0, 1, 2):
+        print(f"launch shape {cfg}, precision {precision}", flush=True)
+        sim = hip.InamuroHIP(P, precision=precision)
+        sim.set_poisson_launch(cfg)
+        assert sim.poisson_launch() == -1  # chosen or fixed at the first solve
+        sim.upload_fields(start)
+        iters = sim.step(3)
+        used = sim.poisson_launch()
+        assert used == cfg if cfg >= 0 else 0 <= used <= 2
+        out = sim.downloadFieldsToCPU()
+        out.update(sim.download_dists())
+        outs[cfg] = (iters, out)
+        sim.close()
+    it0, ref = outs[0]
+    assert max(it0) > 100
+    for cfg, (its, out) in outs.items():
+        assert its == it0, cfg
+        for k in ref:
+            assert np.array_equal(out[k], ref[k]), (cfg, k)
+    with pytest.raises(ValueError):
+        hip.InamuroHIP(P, precision=precision).set_poisson_launch(3)
