@@ -137,6 +137,7 @@ def run_precision(hip, wl, precision, steps, warmup):
     )
     out["fields"] = sim.downloadFieldsToCPU()
     out["mass"] = sim.mass()  # device FP64 sum of phi
+    out["poisson_launch"] = sim.poisson_launch()  # launch shape the context chose
     sim.close()
     return out
 
@@ -252,12 +253,13 @@ def run_adjoint(hip, steps, checkpoint, fd_directions=0, h=1e-6):
     return rec
 
 
-def run_adjoint_falling(hip, steps, checkpoint, state, fd=False, h=1e-6):
+def run_adjoint_falling(hip, steps, checkpoint, state, fd=False, h=1e-6, iterations=None):
     """The falling droplet's adjoint: dJ/dk_g (and dJ/dphi0) of its centroid
     height after `steps` steps from the release, on the full grid with the
     walls and gravity, the adjoint Poisson state in `state` precision.  Cost
     ratio as in run_adjoint; with fd, central differences of the same
-    frozen-count map in k_g."""
+    frozen-count map in k_g.  `iterations` freezes given Poisson counts (a
+    short probe of the adjoint kernels at full size for counter runs)."""
     wl = WORKLOADS["falling_droplet"]
     lz, ly, lx = wl["shape"]
 
@@ -277,6 +279,8 @@ def run_adjoint_falling(hip, steps, checkpoint, state, fd=False, h=1e-6):
     # the adjoint's start: rho linear in phi0, at rest
     sim.initFromCPU(fei=phi0, rho=sim.initial_rho(phi0), u=0.0, v=0.0, w=0.0, p=0.0)
     sim.synchronize()
+    if iterations is not None:
+        sim.set_iteration_schedule(iterations)
     t0 = time.perf_counter()
     for k in range(steps):
         sim.performTimeStepGPU(1)
@@ -429,6 +433,9 @@ def main(argv=None):
                     help="with --adjoint-falling: central differences in k_g")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one short solve per precision (for rocprofv3 --pmc runs) and exit")
+    ap.add_argument("--adjoint-probe", action="store_true",
+                    help="the falling droplet's adjoint at full size over 2 steps of 100 and 300 frozen "
+                         "Poisson iterations (FP32 state; for rocprofv3 runs of the adjoint kernels) and exit")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: the time step is not sharded across GPUs in this "
@@ -440,6 +447,10 @@ def main(argv=None):
     global GRAPHS
     GRAPHS = not args.no_graphs
 
+    if args.adjoint_probe:
+        rec = run_adjoint_falling(hip, 2, 2, "fp32", iterations=[100, 300])
+        print(json.dumps(rec), flush=True)
+        return 0
     if args.adjoint_falling:
         rec = run_adjoint_falling(hip, args.adjoint_falling, min(200, args.adjoint_falling),
                                   args.adjoint_falling_state, fd=args.adjoint_falling_fd)
@@ -488,6 +499,10 @@ def main(argv=None):
                          + ", density ratio 1000, Poisson tol 1e-3 / check 100, steps after a cold start"),
             "grid": [lx, ly, lz], "radius": wl["R"], "precision": head["precision"],
             "poisson_iterations_per_step": head["iterations"],
+            # launch shape of the Poisson kernel each context chose by timing
+            # (fp64: 0 = 256 threads non-temporal, 1 = 256, 2 = 768; mixed:
+            # 0 = 1024 non-temporal, 1 = 512 non-temporal, 2 = 512)
+            "poisson_launch": {p: results[p]["poisson_launch"] for p in results},
         },
         "roofline": {
             "bound": "hbm",

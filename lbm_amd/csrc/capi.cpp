@@ -943,7 +943,7 @@ void adj_forward_block(Ctx& c, AdjWork& a, const double* p_start, double* start,
     for (uint64_t k = a0 + 1; k <= a1; ++k) {
         double* out = a.slot(c, k - a0);
         lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, plast, D<double>(c.tau), D<double>(c.src), c.dst,
-                               0);
+                               c.push_cfg);
         if (k % ci == 0) {
             double* pk = a.pstore + ((k - a0) / ci) * n;
             lbm::f64::gauge(c.stream, c.g, c.ph, out, pk, c.p_prev, c.partials, c.dst);
@@ -983,7 +983,7 @@ void adj_forward_snaps(Ctx& c, AdjWork& a, const double* p_start, const double* 
         const SnapView v = snap_view(c, mark ? marks[m].second : a.slot(c, 1));
         double* out = mark ? v.h : a.slot(c, 1 + (k & 1));
         lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, plast, D<double>(c.tau), D<double>(c.src), c.dst,
-                               0);
+                               c.push_cfg);
         if (k % ci == 0) {
             double* pk = mark ? v.p : a.pstore + ((k / ci) & 1) * n;
             lbm::f64::gauge(c.stream, c.g, c.ph, out, pk, c.p_prev, c.partials, c.dst);
@@ -1029,6 +1029,11 @@ struct StepRev {
     lbm_adjoint_stats_t* stats;
 };
 
+// Threads per block of the adjoint Poisson kernel: it streams like the
+// forward push kernel (the same loads and x-shifted stores, in reverse), so it
+// follows the forward kernel's timed choice, 768 threads or 256.
+int adj_block_size(const Ctx& c) { return c.push_cfg == 2 ? 768 : lbm::BLOCK; }
+
 // Block j of the solve (iterations jL+1 .. min(nit, (j+1)L)) from snapshot
 // `start` of check jL (nullptr for j = 0): recompute it recording every
 // iterate, then run the adjoints of its iterations, last first.
@@ -1065,7 +1070,7 @@ void adj_block(Ctx& c, AdjWork& a, const StepRev& r, uint64_t j, double* start)
         }
         lbm::adj::poisson(s, g, c.ph, src, mode, c.adj_state_f32, a.Lb[a.lcur], a.lpf, a.as, raw, pin,
                           shift, D<double>(c.tau), a.lS, a.ltau, k == 1 ? nullptr : a.Lb[1 - a.lcur],
-                          k == 1 ? a.lps : nullptr, a.parts);
+                          k == 1 ? a.lps : nullptr, a.parts, adj_block_size(c));
         if (k > 1) a.lcur = 1 - a.lcur;
         if ((k - 1) % ci == 0)  // a check interval of the solve reversed
             lbm::adj::converge(s, g, a.lS, a.lS_prev, nit - k + 1, c.tol, k + ci - 1 == nit ? 1 : 0, a.parts,

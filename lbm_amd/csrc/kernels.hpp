@@ -156,10 +156,12 @@ void correct(hipStream_t s, const Grid& g, const Phys& ph, const double* p, cons
 // Bin / Bout are [q][n] carriers of the adjoint, float when state_f32 is set
 // and double otherwise; the first iteration (mode H_EQUILIBRIUM) writes
 // lp_start instead of Bout.
+// block: threads per block of the non-gauged iterations, BLOCK or 768 (the
+// results do not depend on it).
 void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, bool state_f32,
              const void* Bin, const double* lpf, AdjState* as, const double* raw, const double* pin,
              const double* shift, const double* tau, double* lS, double* ltau, void* Bout,
-             double* lp_start, double* parts);
+             double* lp_start, double* parts, int block);
 void source(hipStream_t s, const Grid& g, const Phys& ph, const double* fei, const double* rho,
             const double* ltau, const double* lS, const double* lfei, AdjState* as, double* lrho,
             double* lps, double* lmac, double* parts);

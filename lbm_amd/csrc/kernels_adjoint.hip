@@ -302,8 +302,8 @@ enum { SRC_PLAIN = 0, SRC_LAZY = 1, SRC_INIT = 2 };
 // TB is the storage type of the carried adjoint Bin/Bout: double, or float
 // for the FP32 adjoint state (arithmetic, lS, ltau and the gauge partials stay
 // FP64 either way).
-template <int SRC, int MODE, typename TB>
-__global__ __launch_bounds__(BLOCK) void k_adj_poisson(
+template <int SRC, int MODE, typename TB, int PB>
+__global__ __launch_bounds__(PB) void k_adj_poisson(
     Grid g, const TB* __restrict__ Bin, const double* __restrict__ lpf,
     const AdjState* __restrict__ as, const double* __restrict__ raw,
     const double* __restrict__ pin, const double* __restrict__ shift_ptr,
@@ -311,7 +311,9 @@ __global__ __launch_bounds__(BLOCK) void k_adj_poisson(
     TB* __restrict__ Bout, double* __restrict__ lp_start, double* __restrict__ parts)
 {
     __shared__ double lds[4];
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    // the gauged iteration's block sum assumes BLOCK threads
+    static_assert(MODE != H_GAUGED || PB == BLOCK, "gauged adjoint iterations run BLOCK-thread blocks");
+    const int64_t c = static_cast<int64_t>(blockIdx.x) * PB + threadIdx.x;
     double part = 0.0;
     if (c < g.n) {
         int x, y, z;
@@ -770,12 +772,23 @@ template <typename TB>
 static void poisson_t(hipStream_t s, const Grid& g, int src, int mode, const TB* Bin,
                       const double* lpf, AdjState* as, const double* raw, const double* pin,
                       const double* shift, const double* tau, double* lS, double* ltau, TB* Bout,
-                      double* lp_start, double* parts)
+                      double* lp_start, double* parts, int block)
 {
-    const int nb = grid_blocks(g.n);
-#define LBM_ADJ_LAUNCH(S, M)                                                                       \
-    k_adj_poisson<S, M, TB><<<nb, BLOCK, 0, s>>>(g, Bin, lpf, as, raw, pin, shift, tau, lS, ltau, Bout, \
-                                                 lp_start, parts)
+    // The gauged iterations reduce block sums over BLOCK threads (their count
+    // and order fixed by the grid); the others may use wider blocks.
+#define LBM_ADJ_AT(S, M, PB)                                                                          \
+    k_adj_poisson<S, M, TB, PB><<<static_cast<int>((g.n + PB - 1) / PB), PB, 0, s>>>(                 \
+        g, Bin, lpf, as, raw, pin, shift, tau, lS, ltau, Bout, lp_start, parts)
+#define LBM_ADJ_LAUNCH(S, M)                                         \
+    do {                                                             \
+        if constexpr (M != H_GAUGED) {                               \
+            if (block == 768) {                                      \
+                LBM_ADJ_AT(S, M, 768);                               \
+                break;                                               \
+            }                                                        \
+        }                                                            \
+        LBM_ADJ_AT(S, M, BLOCK);                                     \
+    } while (0)
 #define LBM_ADJ_MODES(S)                                         \
     do {                                                         \
         if (mode == H_EQUILIBRIUM) LBM_ADJ_LAUNCH(S, H_EQUILIBRIUM); \
@@ -787,19 +800,20 @@ static void poisson_t(hipStream_t s, const Grid& g, int src, int mode, const TB*
     else LBM_ADJ_MODES(SRC_PLAIN);
 #undef LBM_ADJ_MODES
 #undef LBM_ADJ_LAUNCH
+#undef LBM_ADJ_AT
 }
 
 void poisson(hipStream_t s, const Grid& g, const Phys& ph, int src, int mode, bool state_f32,
              const void* Bin, const double* lpf, AdjState* as, const double* raw, const double* pin,
              const double* shift, const double* tau, double* lS, double* ltau, void* Bout,
-             double* lp_start, double* parts)
+             double* lp_start, double* parts, int block)
 {
     if (state_f32)
         poisson_t(s, g, src, mode, static_cast<const float*>(Bin), lpf, as, raw, pin, shift, tau, lS,
-                  ltau, static_cast<float*>(Bout), lp_start, parts);
+                  ltau, static_cast<float*>(Bout), lp_start, parts, block);
     else
         poisson_t(s, g, src, mode, static_cast<const double*>(Bin), lpf, as, raw, pin, shift, tau, lS,
-                  ltau, static_cast<double*>(Bout), lp_start, parts);
+                  ltau, static_cast<double*>(Bout), lp_start, parts, block);
     if (mode == H_GAUGED) k_adj_lam_finalize<<<1, BLOCK, 0, s>>>(parts, grid_blocks(g.n), ph.cell_count, 0, as);
 }
 

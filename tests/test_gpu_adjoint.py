@@ -267,3 +267,22 @@ def test_set_poisson_validates_every_argument_first():
     got = sim.step(3)
     assert got[:2] == [200, 300]
     assert got == ref.step(3)  # the third step converges with the default interval and tolerance
+
+
+def test_launch_shape_does_not_change_the_gradient():
+    """The forward kernel's launch shape (which the recomputation and the
+    adjoint Poisson kernel follow: 256- or 768-thread blocks) changes no bit
+    of J or the gradients."""
+    P = params(lx=16, ly=12, lz=12, rho_L=1000.0, DD=8.0)
+    phi0, target, _ = case(P, seed=5)
+    out = []
+    for cfg in (0, 1, 2):
+        announce(f"adjoint with launch shape {cfg}")
+        sim = hip.InamuroHIP(P)
+        sim.set_poisson_launch(cfg)
+        out.append(sim.adjoint(phi0, target, 3, checkpoint_interval=2))
+        assert sim.poisson_launch() == cfg
+        sim.close()
+    for J, g, gk in out[1:]:
+        assert J == out[0][0] and gk == out[0][2]
+        assert np.array_equal(g, out[0][1])
