@@ -239,16 +239,22 @@ def run_adjoint(hip, steps, checkpoint, fd_directions=0, h=1e-6):
             rows.append(dict(fd=fd, adjoint=ad, rel=abs(fd - ad) / abs(fd)))
             print(f"adjoint fd direction {i}: fd {fd:.10e} adjoint {ad:.10e} rel {rows[-1]['rel']:.2e}",
                   file=sys.stderr, flush=True)
-        jk = []
-        for kg in (P.k_g + h, P.k_g - h):
+        # k_g at +-h and +-2h: the two-point central difference and the
+        # five-point one, (8 (J(+h) - J(-h)) - (J(+2h) - J(-2h))) / (12 h),
+        # whose truncation error is O(h^4) instead of O(h^2)
+        jk = {}
+        for m in (1, -1, 2, -2):
+            kg = P.k_g + m * h
             x = hip.InamuroHIP(dataclasses.replace(P, k_g=kg))
             x.setUsePoissonGraph(GRAPHS)
-            jk.append(x.objective(phi0, target, its))
-            print(f"adjoint fd k_g = {kg:.6e}: J {jk[-1]:.12e}", file=sys.stderr, flush=True)
+            jk[m] = x.objective(phi0, target, its)
+            print(f"adjoint fd k_g = {kg:.6e}: J {jk[m]:.12e}", file=sys.stderr, flush=True)
             x.close()
-        fdk = (jk[0] - jk[1]) / (2 * h)
+        fdk = (jk[1] - jk[-1]) / (2 * h)
+        fdk4 = (8.0 * (jk[1] - jk[-1]) - (jk[2] - jk[-2])) / (12.0 * h)
         rec["finite_differences"] = dict(h=h, phi0_directions=rows, max_rel_phi0=max(r["rel"] for r in rows),
-                                         kg=dict(fd=fdk, adjoint=gk, rel=abs(fdk - gk) / abs(fdk)))
+                                         kg=dict(fd=fdk, adjoint=gk, rel=abs(fdk - gk) / abs(fdk)),
+                                         kg_five_point=dict(fd=fdk4, adjoint=gk, rel=abs(fdk4 - gk) / abs(fdk4)))
     sim.close()
     return rec
 
@@ -501,7 +507,7 @@ def main(argv=None):
             "poisson_iterations_per_step": head["iterations"],
             # launch shape of the Poisson kernel each context chose by timing
             # (fp64: 0 = 256 threads non-temporal, 1 = 256, 2 = 768; mixed:
-            # 0 = 1024 non-temporal, 1 = 512 non-temporal, 2 = 512)
+            # 0 = 1024 non-temporal, 1 = 512 non-temporal, 2 = 512, 3 = 1024)
             "poisson_launch": {p: results[p]["poisson_launch"] for p in results},
         },
         "roofline": {
@@ -537,7 +543,10 @@ def main(argv=None):
     for prec in results:
         traffic = os.path.join(ROOT, "profiles", f"traffic_{prec}_{args.workload}.json")
         if os.path.exists(traffic):
-            tb = json.load(open(traffic)).get("bytes_per_launch")
+            rec = json.load(open(traffic))
+            # the record of the launch shape this run's context chose
+            shape = rec.get("by_launch", {}).get(str(results[prec]["poisson_launch"]))
+            tb = shape["bytes_per_launch"] if shape else rec.get("bytes_per_launch")
             if prec == head["precision"]:
                 line["roofline"]["traffic"] = tb
             else:

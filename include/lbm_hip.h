@@ -222,9 +222,10 @@ int lbm_set_poisson_graph(void* ctx, int enabled);
 
 /* Launch shape of the Poisson iteration kernel (no counterpart in the
  * reference): -1, the default, times the candidates on the context's own
- * buffers before the first solve and keeps the fastest for this grid; 0..2
- * fixes one (FP64: 256 threads with non-temporal loads of h, 256 plain, 768
- * plain; mixed: 1024 threads with non-temporal loads, 512 with, 512 plain).
+ * buffers before the first solve and keeps the fastest for this grid; a
+ * candidate's index fixes it (FP64, 0..2: 256 threads with non-temporal loads
+ * of h, 256 plain, 768 plain; mixed, 0..3: 1024 threads with non-temporal
+ * loads, 512 with, 512 plain, 1024 plain).
  * Every choice gives bit-identical results; only the speed differs with the
  * grid's shape.  lbm_get_poisson_launch reports the one in use (-1 before the
  * first solve when choosing automatically). */

@@ -1660,8 +1660,10 @@ int lbm_set_gravity(void* ctx, double gz)
 int lbm_set_poisson_launch(void* ctx, int config)
 {
     return guarded(ctx, [&](Ctx& c) {
-        if (config < -1 || config >= lbm::f64::PUSH_CONFIGS)
-            throw ApiError{LBM_ERR_ARG, "Poisson launch config must be -1 (automatic) or 0..2"};
+        const int ncfg = c.mixed ? lbm::mx::PUSH_CONFIGS : lbm::f64::PUSH_CONFIGS;
+        if (config < -1 || config >= ncfg)
+            throw ApiError{LBM_ERR_ARG, "Poisson launch config must be -1 (automatic) or 0.." +
+                                            std::to_string(ncfg - 1) + " for this precision"};
         c.push_request = config;
         c.push_cfg = -1;
         clear_graphs(c);

@@ -211,7 +211,7 @@ void collide(hipStream_t s, int mode, const Grid& g, const PhysF& ph, const floa
 void macro(hipStream_t s, const Grid& g, const PhysF& ph, float* fpost, const float* gpost,
            const float* ux, const float* vy, const float* wz, DevState* st, float* fei,
            float* rho, float* u, float* v, float* w, float* omega, float* src3, double* partials);
-constexpr int PUSH_CONFIGS = 3;
+constexpr int PUSH_CONFIGS = 4;
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const float* hin, float* hout,
                   const float* bbase, const double* pown, float* bout, const float* omega,
                   const float* src3, const DevState* st, int config);

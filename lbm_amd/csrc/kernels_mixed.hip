@@ -671,7 +671,7 @@ void poisson_iter(hipStream_t s, int mode, const Grid& g, const float* hin, floa
                   const float* src3, const DevState* st, int config)
 {
     // PUSH_CONFIGS: 0 = 1024 threads with non-temporal loads, 1 = 512 with
-    // non-temporal loads, 2 = 512 plain.
+    // non-temporal loads, 2 = 512 plain, 3 = 1024 plain.
 #define LBM_MX_PUSH_AT(M, NT, PB)                                                                    \
     k_poisson_push<M, NT, PB><<<static_cast<int>((g.n + PB - 1) / PB), PB, 0, s>>>(                  \
         g, hin, hout, bbase, pown, bout, omega, src3, st)
@@ -681,6 +681,8 @@ void poisson_iter(hipStream_t s, int mode, const Grid& g, const float* hin, floa
             LBM_MX_PUSH_AT(M, 1, 512);                                                         \
         else if (config == 2)                                                                  \
             LBM_MX_PUSH_AT(M, 0, 512);                                                         \
+        else if (config == 3)                                                                  \
+            LBM_MX_PUSH_AT(M, 0, 1024);                                                        \
         else                                                                                   \
             LBM_MX_PUSH_AT(M, 1, 1024);                                                        \
     } while (0)
