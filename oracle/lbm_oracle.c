@@ -78,6 +78,7 @@ struct orc_sim {
     int pressure_solve_call_count;
     uint64_t* schedule;                 /* frozen per-step iteration counts (extension) */
     int schedule_len, schedule_pos;
+    orc_poisson_accel accel;            /* reference GPU Poisson accelerators */
     orc_poisson_diag diag;
     char err[256];
 };
@@ -164,6 +165,7 @@ orc_sim* orc_create(const orc_params* params)
     s->tol = 1.0e-3;
     s->iter_cap = 0;
     s->dual = 0;
+    orc_default_poisson_accel(&s->accel);
     return s;
 }
 
@@ -687,12 +689,19 @@ void orc_poisson_iterations(orc_sim* s, int k)
  * reference's function-static pressure_prev, zero-initialised once per
  * simulation and kept across steps. */
 static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p);
+static int solve_pressure_poisson_accel(orc_sim* s);
+static int accel_active(const orc_sim* s);
 
-static int solve_pressure_poisson(orc_sim* s) { return solve_pressure_poisson_rec(s, NULL, NULL); }
+static int solve_pressure_poisson(orc_sim* s)
+{
+    if (accel_active(s)) return solve_pressure_poisson_accel(s);
+    return solve_pressure_poisson_rec(s, NULL, NULL);
+}
 
-/* rec_h / rec_p (extension, for the adjoint): when non-NULL, receive the h and
- * p each iteration starts from (after any gauge), iteration-major. */
-static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p)
+/* Start of a solve (CPU/Inamuro.cpp:624-668, GPU/InamuroCUDA.cu:1829-1925):
+ * diagnostics reset, source statistics, projection of the source mean out of
+ * u_x, and the gauge target mean(p).  0 = ok. */
+static int poisson_prologue(orc_sim* s, double* gauge_target_out)
 {
     const size_t n = s->n;
     ++s->pressure_solve_call_count;
@@ -704,6 +713,8 @@ static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p)
     d->fixed_point_h_l1 = d->fixed_point_h_scale = d->fixed_point_h_relative = NAN;
     d->fixed_point_p_l1 = d->fixed_point_p_scale = d->fixed_point_p_relative = NAN;
     d->fixed_point_relative = NAN;
+    d->block_low_frequency_fraction = NAN;
+    d->anderson_alpha = NAN;
 
     const double cell_count = (double)s->lx * s->ly * s->lz;
     double divergence_sum = 0.0, pressure_sum = 0.0, divergence_abs_sum = 0.0;
@@ -733,6 +744,20 @@ static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p)
     double projected_sum = 0.0;
     for (size_t i = 0; i < n; ++i) projected_sum += ux[i] + vy[i] + wz[i];
     d->projected_source_mean = -projected_sum / (3.0 * cell_count);
+    *gauge_target_out = gauge_target;
+    return 0;
+}
+
+/* rec_h / rec_p (extension, for the adjoint): when non-NULL, receive the h and
+ * p each iteration starts from (after any gauge), iteration-major. */
+static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p)
+{
+    const size_t n = s->n;
+    orc_poisson_diag* d = &s->diag;
+    const double cell_count = (double)s->lx * s->ly * s->lz;
+    double gauge_target = 0.0;
+    if (poisson_prologue(s, &gauge_target) != 0) return -1;
+    double* p = A(s, ORC_P);
 
     const uint64_t forced = s->schedule_pos < s->schedule_len ? s->schedule[s->schedule_pos] : 0;
     uint64_t iter = 0;
@@ -783,6 +808,452 @@ static int solve_pressure_poisson_rec(orc_sim* s, double* rec_h, double* rec_p)
     d->iteration_count = iter;
     if (forced) ++s->schedule_pos;
     return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* The reference GPU solver's Poisson accelerators                     */
+/* ------------------------------------------------------------------ */
+/* Restated from the CUDA kernels (GPU/InamuroCUDA.cu:503-583, 732-1013,     */
+/* 1198-1323) and their driver loop doPressurePoisson (:1816-2349).  The CPU */
+/* reference has none of them.  Global sums run in this oracle's x-outer     */
+/* order (the reference uses atomics, i.e. no fixed order).                  */
+
+void orc_default_poisson_accel(orc_poisson_accel* a)
+{
+    memset(a, 0, sizeof(*a));
+    a->variant = ORC_POISSON_SPLIT;
+    a->pressure_relax_scale = 1.0;
+    a->fixed_point_relax = 1.0;
+    a->scalar_source_scale = 2.0;  /* GPU/InamuroCUDA.hpp:152-157 */
+    a->source_aware_hh_scale = 1.0;
+    a->anderson_beta_max = 1.0;
+    a->two_grid_strength = 0.5;
+}
+
+/* The setters' validation, GPU/InamuroCUDA.cu:2488-2540. */
+int orc_set_poisson_accel(orc_sim* s, const orc_poisson_accel* a)
+{
+    if (a->variant < ORC_POISSON_SPLIT || a->variant > ORC_POISSON_SCALAR) {
+        set_err(s, "--poisson must be split, fused, onepass, or scalar");
+        return -1;
+    }
+    if (!(a->pressure_relax_scale > 0.0)) { set_err(s, "pressure relaxation scale must be positive"); return -1; }
+    if (!(a->fixed_point_relax > 0.0)) { set_err(s, "Poisson fixed-point relaxation must be positive"); return -1; }
+    if (!(a->anderson_beta_max >= 0.0)) { set_err(s, "Poisson Anderson beta max must be non-negative"); return -1; }
+    if (!(a->two_grid_strength >= 0.0)) { set_err(s, "Poisson two-grid strength must be non-negative"); return -1; }
+    s->accel = *a;
+    return 0;
+}
+
+static int accel_active(const orc_sim* s)
+{
+    const orc_poisson_accel* a = &s->accel;
+    return a->variant != ORC_POISSON_SPLIT || a->fused_boundary || a->pressure_relax_scale != 1.0 ||
+           a->fixed_point_relax != 1.0 || a->source_aware_hh_init || a->anderson_m1 || a->two_grid ||
+           a->spatial_diagnostics;
+}
+
+/* collisionPressureKernel with the relaxation scale (GPU/InamuroCUDA.cu:
+ * 503-537): h <- h - s (h - E p) / tau_h - (E / 3) div, in place. */
+static void collision_p_scaled(orc_sim* s, double* hdist, double scale)
+{
+    const size_t n = s->n;
+    const double* rho = A(s, ORC_RHO);
+    const double* p = A(s, ORC_P);
+    const double* ux = A(s, ORC_U_X);
+    const double* vy = A(s, ORC_V_Y);
+    const double* wz = A(s, ORC_W_Z);
+#pragma omp parallel for num_threads(s->nthreads) schedule(static) if (s->nthreads > 1)
+    for (size_t i = 0; i < n; ++i) {
+        const double div_u = ux[i] + vy[i] + wz[i];
+        const double tauh = 1.0 / rho[i] + 0.5;
+        for (int k = 0; k < Q; ++k) {
+            const double hequ = EW[k] * p[i];
+            double* hk = &hdist[(size_t)k * n + i];
+            *hk = *hk - scale * (*hk - hequ) / tauh - (EW[k] / 3.0) * div_u;
+        }
+    }
+}
+
+/* The image of cell (x, y, z) under the pull form of one iteration: the
+ * collision of collisionPressureStreamKernel (GPU/InamuroCUDA.cu:539-583)
+ * evaluated at the source cell x - c_q with periodic wrap, then the x-slip
+ * overwrite of boundaryAndComputePressureKernel (:732-776), which
+ * collisionStreamBoundaryPressureKernel (:778-846) applies the same way. */
+static void accel_image_cell(const orc_sim* s, const double* h, const double* p, int x, int y, int z,
+                             double* img)
+{
+    const size_t n = s->n;
+    const double* rho = A(s, ORC_RHO);
+    const double* ux = A(s, ORC_U_X);
+    const double* vy = A(s, ORC_V_Y);
+    const double* wz = A(s, ORC_W_Z);
+    const double scale = s->accel.pressure_relax_scale;
+    for (int q = 0; q < Q; ++q) {
+        const int xs = (x - EX[q] + s->lx) % s->lx;
+        const int ys = (y - EY[q] + s->ly) % s->ly;
+        const int zs = (z - EZ[q] + s->lz) % s->lz;
+        const size_t src = IX(s, xs, ys, zs);
+        const double tauh = 1.0 / rho[src] + 0.5;
+        const double hequ = EW[q] * p[src];
+        const double div_u = ux[src] + vy[src] + wz[src];
+        const double h_old = h[(size_t)q * n + src];
+        img[q] = h_old - scale * (h_old - hequ) / tauh - (EW[q] / 3.0) * div_u;
+    }
+    if (x == 0) {
+        for (int r = 0; r < 5; ++r) img[SLIP_POS[r]] = img[SLIP_NEG[r]];
+    } else if (x == s->lx - 1) {
+        for (int r = 0; r < 5; ++r) img[SLIP_NEG[r]] = img[SLIP_POS[r]];
+    }
+}
+
+/* pressure_bounce_direction, GPU/InamuroCUDA.cu:848-866: at the x walls a
+ * neighbour beyond the wall is replaced by its mirror image. */
+static int pressure_bounce_direction(int q, int x, int lx)
+{
+    if (x == 0) {
+        for (int r = 0; r < 5; ++r)
+            if (q == SLIP_POS[r]) return SLIP_NEG[r];
+    } else if (x == lx - 1) {
+        for (int r = 0; r < 5; ++r)
+            if (q == SLIP_NEG[r]) return SLIP_POS[r];
+    }
+    return q;
+}
+
+/* Work arrays of one accelerated solve. */
+typedef struct accel_work {
+    double* hn;        /* next h (15 n)                              */
+    double* pn;        /* next p (n)                                 */
+    double* prev_res;  /* Anderson: previous residual (15 n)         */
+    double* prev_img;  /* Anderson: previous image (15 n)            */
+    double* blocks;    /* 4^3 block sums of p - p_prev               */
+    int has_hist;
+} accel_work;
+
+/* One iteration of the selected variant: h and p of the simulation advance
+ * in place (GPU/InamuroCUDA.cu:2216-2323). */
+static void accel_iteration(orc_sim* s, accel_work* w)
+{
+    const int lx = s->lx, ly = s->ly, lz = s->lz;
+    const size_t n = s->n;
+    const orc_poisson_accel* a = &s->accel;
+    double* h = s->h;
+    double* p = A(s, ORC_P);
+    const double* rho = A(s, ORC_RHO);
+    const double* ux = A(s, ORC_U_X);
+    const double* vy = A(s, ORC_V_Y);
+    const double* wz = A(s, ORC_W_Z);
+
+    if (a->variant == ORC_POISSON_SCALAR) {
+        /* scalarPressureJacobiKernel (:868-908). */
+        const double scale = a->scalar_source_scale;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static) if (s->nthreads > 1)
+        for (int x = 0; x < lx; ++x)
+            for (int y = 0; y < ly; ++y)
+                for (int z = 0; z < lz; ++z) {
+                    double neighbor_sum = 0.0;
+                    for (int q = 1; q < Q; ++q) {
+                        const int qb = pressure_bounce_direction(q, x, lx);
+                        const int xs = (x - EX[qb] + lx) % lx;
+                        const int ys = (y - EY[qb] + ly) % ly;
+                        const int zs = (z - EZ[qb] + lz) % lz;
+                        neighbor_sum += p[IX(s, xs, ys, zs)];
+                    }
+                    const size_t i = IX(s, x, y, z);
+                    const double div_u = ux[i] + vy[i] + wz[i];
+                    const double rhs = scale * rho[i] * div_u;
+                    w->pn[i] = (neighbor_sum - 5.0 * rhs) / 14.0;
+                }
+        memcpy(p, w->pn, sizeof(double) * n);
+        return;
+    }
+
+    if (a->variant == ORC_POISSON_SPLIT) {
+        /* collisionPressureKernel, streamKernel, then slipBounceBackKernel +
+         * computePressureKernel or boundaryAndComputePressureKernel: the same
+         * values either way (:2296-2322). */
+        collision_p_scaled(s, h, a->pressure_relax_scale);
+        stream(s, h);
+        slip_bounce_back(s, h);
+        sum_h_into(s, h, p);
+        return;
+    }
+
+    const int onepass = a->variant == ORC_POISSON_ONEPASS;
+    const double omega = a->fixed_point_relax;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static) if (s->nthreads > 1)
+    for (int x = 0; x < lx; ++x)
+        for (int y = 0; y < ly; ++y)
+            for (int z = 0; z < lz; ++z) {
+                double img[Q];
+                accel_image_cell(s, h, p, x, y, z, img);
+                const size_t i = IX(s, x, y, z);
+                double p_sum = 0.0;
+                for (int q = 0; q < Q; ++q) {
+                    double v = img[q];
+                    if (onepass) {
+                        /* collisionStreamBoundaryPressureKernel (:836-842). */
+                        const double h_old = h[(size_t)q * n + i];
+                        v = h_old + omega * (v - h_old);
+                    }
+                    w->hn[(size_t)q * n + i] = v;
+                    p_sum += v;
+                }
+                w->pn[i] = p_sum;
+            }
+
+    if (onepass && a->anderson_m1) {
+        const size_t nv = (size_t)Q * n;
+        if (!w->has_hist) {
+            /* andersonStoreHistoryKernel (:930-942). */
+            for (size_t k = 0; k < nv; ++k) {
+                w->prev_res[k] = w->hn[k] - h[k];
+                w->prev_img[k] = w->hn[k];
+            }
+            w->has_hist = 1;
+        } else {
+            /* andersonM1DotKernel (:944-981) and the host's coefficient
+             * (:2254-2266). */
+            double num = 0.0, den = 0.0;
+            for (size_t k = 0; k < nv; ++k) {
+                const double residual = w->hn[k] - h[k];
+                const double delta = residual - w->prev_res[k];
+                num += w->prev_res[k] * delta;
+                den += delta * delta;
+            }
+            double alpha = 1.0;
+            if (den > 1.0e-300) alpha = -num / den;
+            if (!isfinite(alpha)) alpha = 1.0;
+            const double lo = -a->anderson_beta_max;
+            const double hi = 1.0 + a->anderson_beta_max;
+            if (alpha > hi) alpha = hi;
+            if (alpha < lo) alpha = lo;
+            s->diag.anderson_alpha = alpha;
+            /* andersonM1ApplyKernel (:983-1013). */
+#pragma omp parallel for num_threads(s->nthreads) schedule(static) if (s->nthreads > 1)
+            for (size_t i = 0; i < n; ++i) {
+                double p_sum = 0.0;
+                for (int q = 0; q < Q; ++q) {
+                    const size_t k = (size_t)q * n + i;
+                    const double current_image = w->hn[k];
+                    const double accelerated = (1.0 - alpha) * w->prev_img[k] + alpha * current_image;
+                    w->prev_res[k] = current_image - h[k];
+                    w->prev_img[k] = current_image;
+                    w->hn[k] = accelerated;
+                    p_sum += accelerated;
+                }
+                w->pn[i] = p_sum;
+            }
+        }
+    }
+    memcpy(h, w->hn, sizeof(double) * (size_t)Q * n);
+    memcpy(p, w->pn, sizeof(double) * n);
+}
+
+#define ACCEL_BLOCK 4 /* spatial_block_size, GPU/InamuroCUDA.cu:2061 */
+
+/* check_residual of the accelerated loop (GPU/InamuroCUDA.cu:1960-2160)
+ * without the dual residual, which the reference refuses with any
+ * accelerator.  Returns 1 when converged, 0 when not, -1 on error. */
+static int accel_check(orc_sim* s, accel_work* w, double gauge_target, uint64_t iter)
+{
+    const int lx = s->lx, ly = s->ly, lz = s->lz;
+    const size_t n = s->n;
+    const orc_poisson_accel* a = &s->accel;
+    orc_poisson_diag* d = &s->diag;
+    const double cell_count = (double)lx * ly * lz;
+    double* p = A(s, ORC_P);
+    double* pp = A(s, ORC_P_PREV);
+    double* h = s->h;
+
+    /* pressure gauge (:1963-1970) on p and h */
+    double cur = 0.0;
+    for (size_t i = 0; i < n; ++i) cur += p[i];
+    const double shift = cur / cell_count - gauge_target;
+    if (!isfinite(shift)) { set_err(s, "pressure gauge is NaN/Inf"); return -1; }
+    if (fabs(shift) > d->pressure_gauge_max_shift) d->pressure_gauge_max_shift = fabs(shift);
+    for (size_t i = 0; i < n; ++i) {
+        p[i] -= shift;
+        for (int k = 0; k < Q; ++k) h[(size_t)k * n + i] -= EW[k] * shift;
+    }
+
+    /* relative change, with 4^3 block sums of p - p_prev when the two-grid
+     * correction or the spatial diagnostics need them
+     * (pressureErrorSpatialKernel, :1198-1248) */
+    const int need_blocks = a->two_grid || a->spatial_diagnostics;
+    const int bx_count = (lx + ACCEL_BLOCK - 1) / ACCEL_BLOCK;
+    const int by_count = (ly + ACCEL_BLOCK - 1) / ACCEL_BLOCK;
+    const int bz_count = (lz + ACCEL_BLOCK - 1) / ACCEL_BLOCK;
+    const int block_count = bx_count * by_count * bz_count;
+    if (need_blocks) memset(w->blocks, 0, sizeof(double) * (size_t)block_count);
+    double err1 = 0.0, err2 = 0.0;
+    for (int x = 0; x < lx; ++x)
+        for (int y = 0; y < ly; ++y)
+            for (int z = 0; z < lz; ++z) {
+                const size_t i = IX(s, x, y, z);
+                const double current = p[i];
+                const double delta = current - pp[i];
+                err1 += fabs(delta);
+                err2 += fabs(current);
+                if (need_blocks) {
+                    const int b = ((z / ACCEL_BLOCK) * by_count + y / ACCEL_BLOCK) * bx_count + x / ACCEL_BLOCK;
+                    w->blocks[b] += delta;
+                }
+                pp[i] = current;
+            }
+    d->pressure_l1_delta = err1;
+    d->pressure_l1_norm = err2;
+    if (!isfinite(err1) || !isfinite(err2)) {
+        set_err(s, "Poisson pressure/residual contains NaN/Inf");
+        return -1;
+    }
+    const double rel = (err2 > 0.0) ? (err1 / err2) : ((err1 == 0.0) ? 0.0 : INFINITY);
+    if (!isfinite(rel)) { set_err(s, "Poisson residual is NaN/Inf"); return -1; }
+    d->relative_error = rel;
+    d->iteration_count = iter;
+    const int converged = rel < s->tol;
+    if (need_blocks) {
+        d->block_size = ACCEL_BLOCK;
+        d->block_count = block_count;
+    }
+    if (a->spatial_diagnostics) {
+        /* blockAbsSumKernel (:1250-1275) and the fraction (:2107-2110) */
+        double block_error = 0.0;
+        for (int b = 0; b < block_count; ++b) block_error += fabs(w->blocks[b]);
+        d->block_low_frequency_fraction = (err1 > 0.0) ? (block_error / err1) : 0.0;
+    }
+    if (a->two_grid && !converged && err1 > 0.0) {
+        /* coarsePressureCorrectionKernel (:1277-1323) */
+        const double strength = a->two_grid_strength;
+#pragma omp parallel for num_threads(s->nthreads) schedule(static) if (s->nthreads > 1)
+        for (int x = 0; x < lx; ++x)
+            for (int y = 0; y < ly; ++y)
+                for (int z = 0; z < lz; ++z) {
+                    const int bx = x / ACCEL_BLOCK, by = y / ACCEL_BLOCK, bz = z / ACCEL_BLOCK;
+                    const int b = (bz * by_count + by) * bx_count + bx;
+                    const int x0 = bx * ACCEL_BLOCK, y0 = by * ACCEL_BLOCK, z0 = bz * ACCEL_BLOCK;
+                    const int x1 = x0 + ACCEL_BLOCK < lx ? x0 + ACCEL_BLOCK : lx;
+                    const int y1 = y0 + ACCEL_BLOCK < ly ? y0 + ACCEL_BLOCK : ly;
+                    const int z1 = z0 + ACCEL_BLOCK < lz ? z0 + ACCEL_BLOCK : lz;
+                    int block_cells = (x1 - x0) * (y1 - y0) * (z1 - z0);
+                    if (block_cells < 1) block_cells = 1;
+                    const double correction = strength * w->blocks[b] / (double)block_cells;
+                    const size_t i = IX(s, x, y, z);
+                    const double corrected_p = p[i] + correction;
+                    p[i] = corrected_p;
+                    pp[i] = corrected_p;
+                    for (int q = 0; q < Q; ++q) h[(size_t)q * n + i] += EW[q] * correction;
+                }
+    }
+    return converged;
+}
+
+/* doPressurePoisson with accelerators (GPU/InamuroCUDA.cu:1816-2349). */
+static int solve_pressure_poisson_accel(orc_sim* s)
+{
+    const orc_poisson_accel* a = &s->accel;
+    if (s->dual) {
+        /* GPU/InamuroCUDA.cu:1846-1865 */
+        if (a->variant != ORC_POISSON_SPLIT || a->fused_boundary || a->source_aware_hh_init ||
+            a->anderson_m1 || a->two_grid || a->pressure_relax_scale != 1.0 || a->fixed_point_relax != 1.0)
+            set_err(s, "fixed-point residual requires frozen split+split Poisson with "
+                       "unit relaxation and all accelerators disabled");
+        else
+            set_err(s, "dual-residual exit requires check_interval=100, tolerance=1e-3, "
+                       "poisson_iteration_limit=0, deterministic reductions, no spatial diagnostics");
+        return -1;
+    }
+    if (s->z_walls) { set_err(s, "Poisson accelerators support the reference's periodic z only"); return -1; }
+    const size_t n = s->n;
+    orc_poisson_diag* d = &s->diag;
+    double gauge_target = 0.0;
+    if (poisson_prologue(s, &gauge_target) != 0) return -1;
+
+    accel_work w;
+    memset(&w, 0, sizeof(w));
+    const int bcount = ((s->lx + ACCEL_BLOCK - 1) / ACCEL_BLOCK) * ((s->ly + ACCEL_BLOCK - 1) / ACCEL_BLOCK) *
+                       ((s->lz + ACCEL_BLOCK - 1) / ACCEL_BLOCK);
+    w.hn = (double*)malloc(sizeof(double) * (size_t)Q * n);
+    w.pn = (double*)malloc(sizeof(double) * n);
+    w.blocks = (double*)malloc(sizeof(double) * (size_t)bcount);
+    int ok = w.hn && w.pn && w.blocks;
+    if (a->anderson_m1 && a->variant == ORC_POISSON_ONEPASS) {
+        w.prev_res = (double*)malloc(sizeof(double) * (size_t)Q * n);
+        w.prev_img = (double*)malloc(sizeof(double) * (size_t)Q * n);
+        ok = ok && w.prev_res && w.prev_img;
+    }
+    int rc = 0;
+    if (!ok) { set_err(s, "out of memory"); rc = -1; goto done; }
+
+    if (a->source_aware_hh_init && a->variant != ORC_POISSON_SCALAR) {
+        /* sourceAwareHHInitKernel (:905-928) */
+        const double* rho = A(s, ORC_RHO);
+        const double* ux = A(s, ORC_U_X);
+        const double* vy = A(s, ORC_V_Y);
+        const double* wz = A(s, ORC_W_Z);
+        const double* p = A(s, ORC_P);
+        const double source_scale = a->source_aware_hh_scale;
+        for (size_t i = 0; i < n; ++i) {
+            const double tauh = 1.0 / rho[i] + 0.5;
+            const double div_u = ux[i] + vy[i] + wz[i];
+            const double loc_p = p[i];
+            for (int q = 0; q < Q; ++q) {
+                const double hequ = EW[q] * loc_p;
+                const double source = (EW[q] / 3.0) * div_u;
+                s->h[(size_t)q * n + i] = hequ - source_scale * tauh * source;
+            }
+        }
+    }
+
+    const uint64_t forced = s->schedule_pos < s->schedule_len ? s->schedule[s->schedule_pos] : 0;
+    uint64_t iter = 0;
+    for (;;) {
+        if (iter == UINT64_MAX) { set_err(s, "Poisson iteration counter overflow"); rc = -2; goto done; }
+        ++iter;
+        accel_iteration(s, &w);
+        if (iter % s->check_interval == 0) {
+            const int conv = accel_check(s, &w, gauge_target, iter);
+            if (conv < 0) { rc = -1; goto done; }
+            if (forced) {
+                if (iter >= forced) break;
+                continue;
+            }
+            if (conv) { d->converged = 1; break; }
+            if (s->iter_cap != 0 && iter >= s->iter_cap) {
+                d->hit_max_iterations = 1;
+                d->fallback_count = 1;
+                break;
+            }
+        }
+    }
+    s->last_iteration_count = iter;
+    d->iteration_count = iter;
+    if (forced) ++s->schedule_pos;
+done:
+    free(w.hn); free(w.pn); free(w.blocks); free(w.prev_res); free(w.prev_img);
+    return rc;
+}
+
+int orc_poisson_accel_iterations(orc_sim* s, int k)
+{
+    const size_t n = s->n;
+    accel_work w;
+    memset(&w, 0, sizeof(w));
+    w.hn = (double*)malloc(sizeof(double) * (size_t)Q * n);
+    w.pn = (double*)malloc(sizeof(double) * n);
+    w.prev_res = (double*)malloc(sizeof(double) * (size_t)Q * n);
+    w.prev_img = (double*)malloc(sizeof(double) * (size_t)Q * n);
+    int rc = 0;
+    if (!w.hn || !w.pn || !w.prev_res || !w.prev_img) {
+        set_err(s, "out of memory");
+        rc = -1;
+    } else {
+        s->diag.anderson_alpha = NAN;
+        for (int it = 0; it < k; ++it) accel_iteration(s, &w);
+    }
+    free(w.hn); free(w.pn); free(w.prev_res); free(w.prev_img);
+    return rc;
 }
 
 /* Extension: no-slip z walls and gravity. */
@@ -991,9 +1462,13 @@ static double adj_value(const orc_sim* s, const double* target, double* seed)
     return J;
 }
 
+/* The adjoint transposes the strict split map only. */
+#define ADJ_ACCEL_MSG "the adjoint differentiates the strict split Poisson map; turn the Poisson accelerators off"
+
 int orc_objective(orc_sim* s, const double* phi0, const double* target, int steps,
                   const uint64_t* iters, double* J)
 {
+    if (accel_active(s)) { set_err(s, ADJ_ACCEL_MSG); return -1; }
     double* t = (double*)calloc(s->n, sizeof(double));
     if (!t) { set_err(s, "out of memory"); return -1; }
     if (s->objective == ORC_OBJ_MISFIT) load_canonical(s, ORC_P_PREV, target, t);
@@ -1091,6 +1566,7 @@ enum { GA_UX, GA_UY, GA_UZ, GA_VX, GA_VY, GA_VZ, GA_WX, GA_WY, GA_WZ,
 int orc_adjoint(orc_sim* s, const double* phi0, const double* target, int steps,
                 const uint64_t* iters, double* J, double* grad_phi0, double* grad_kg)
 {
+    if (accel_active(s)) { set_err(s, ADJ_ACCEL_MSG); return -1; }
     const orc_params* P = &s->P;
     const size_t n = s->n;
     const size_t qn = (size_t)Q * n;

@@ -66,7 +66,43 @@ typedef struct orc_poisson_diag {
      * (GPU/InamuroCUDA.cu:2189, 2337). */
     int32_t fallback_count;
     int32_t reserved;
+    /* Poisson accelerators (orc_set_poisson_accel), last check of the solve:
+     * sum_b |sum_{cells of b} (p - p_prev)| / sum |p - p_prev| over the 4^3
+     * blocks b when spatial diagnostics are on, else NaN
+     * (GPU/InamuroCUDA.cu:2078-2110); the block edge and count when block
+     * sums were formed (spatial diagnostics or two-grid), else 0; and the
+     * last Anderson(1) mixing coefficient (NaN when none was computed). */
+    double block_low_frequency_fraction;
+    int32_t block_size;
+    int32_t block_count;
+    double anderson_alpha;
 } orc_poisson_diag;
+
+/* The reference GPU solver's optional Poisson accelerators
+ * (GPU/InamuroCUDA.hpp:60-76, GPU/InamuroCUDA.cu:539-1013, 1198-1323,
+ * 1816-2349).  They leave the strict contract; the CPU reference has none
+ * of them, so this is a restatement of the CUDA kernels' arithmetic. */
+enum {
+    ORC_POISSON_SPLIT = 0,    /* collision_p, stream, slip, getp (the reference CPU)  */
+    ORC_POISSON_FUSED = 1,    /* collisionPressureStreamKernel: pull with collision at the source */
+    ORC_POISSON_ONEPASS = 2,  /* collisionStreamBoundaryPressureKernel: + fixed-point relaxation */
+    ORC_POISSON_SCALAR = 3    /* scalarPressureJacobiKernel: Jacobi sweep on p alone   */
+};
+typedef struct orc_poisson_accel {
+    int32_t variant;                 /* ORC_POISSON_*                                  */
+    int32_t fused_boundary;          /* boundaryAndComputePressureKernel (same values) */
+    double pressure_relax_scale;     /* s in h - s (h - E p) / tau_h - E S / 3 (> 0)    */
+    double fixed_point_relax;        /* onepass: h_old + w (image - h_old) (> 0)        */
+    double scalar_source_scale;      /* scalar: rhs = scale * rho * div                 */
+    int32_t source_aware_hh_init;    /* h = E p - scale tau_h E S / 3 at solve start    */
+    int32_t anderson_m1;             /* Anderson(1) mixing of the onepass images        */
+    double source_aware_hh_scale;
+    double anderson_beta_max;        /* alpha clamped to [-beta, 1 + beta] (>= 0)       */
+    double two_grid_strength;        /* >= 0                                            */
+    int32_t two_grid;                /* coarse correction from 4^3 block sums at checks */
+    int32_t spatial_diagnostics;     /* block_low_frequency_fraction at checks          */
+} orc_poisson_accel;
+
 
 /* Array identifiers for orc_get / orc_set. */
 enum {
@@ -99,6 +135,17 @@ enum {
 };
 
 typedef struct orc_sim orc_sim;
+
+/* Installs the accelerator settings (defaults: split, split boundary, unit
+ * relaxations, scalar scale 2, source-aware scale 1, beta_max 1, two-grid
+ * strength 0.5, everything off).  Validation as the reference's setters
+ * (GPU/InamuroCUDA.cu:2488-2540).  0 = ok. */
+int orc_set_poisson_accel(orc_sim* s, const orc_poisson_accel* a);
+void orc_default_poisson_accel(orc_poisson_accel* a);
+/* Test helper: k iterations of the configured variant from the current h
+ * and p with u_x + v_y + w_z as the source (no projection, gauge or check;
+ * the Anderson history starts empty).  0 = ok. */
+int orc_poisson_accel_iterations(orc_sim* s, int k);
 
 /* Allocates arrays and applies Inamuro::initializeArrays (CPU/Inamuro.cpp:511-548):
  * f=g=h=0, rho=rho_G, fei=fei_min, u=v=w=p=0. Returns NULL on bad sizes. */

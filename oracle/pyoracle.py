@@ -97,6 +97,24 @@ class PoissonDiag(ctypes.Structure):
         ("converged", ctypes.c_int32), ("source_mean", ctypes.c_double),
         ("source_abs_mean", ctypes.c_double), ("projected_source_mean", ctypes.c_double),
         ("fallback_count", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("block_low_frequency_fraction", ctypes.c_double), ("block_size", ctypes.c_int32),
+        ("block_count", ctypes.c_int32), ("anderson_alpha", ctypes.c_double),
+    ]
+
+
+POISSON_VARIANTS = {"split": 0, "fused": 1, "onepass": 2, "scalar": 3}
+
+
+class PoissonAccel(ctypes.Structure):
+    """orc_poisson_accel: the reference GPU solver's Poisson accelerator
+    settings (GPU/InamuroCUDA.hpp:60-76)."""
+    _fields_ = [
+        ("variant", ctypes.c_int32), ("fused_boundary", ctypes.c_int32),
+        ("pressure_relax_scale", ctypes.c_double), ("fixed_point_relax", ctypes.c_double),
+        ("scalar_source_scale", ctypes.c_double), ("source_aware_hh_init", ctypes.c_int32),
+        ("anderson_m1", ctypes.c_int32), ("source_aware_hh_scale", ctypes.c_double),
+        ("anderson_beta_max", ctypes.c_double), ("two_grid_strength", ctypes.c_double),
+        ("two_grid", ctypes.c_int32), ("spatial_diagnostics", ctypes.c_int32),
     ]
 
 
@@ -139,6 +157,9 @@ def lib():
         L.orc_objective.argtypes = [vp, dp, dp, ctypes.c_int, u64p, dp]
         L.orc_adjoint.argtypes = [vp, dp, dp, ctypes.c_int, u64p, dp, dp, dp]
         L.orc_set_objective.argtypes = [vp, ctypes.c_int]
+        L.orc_set_poisson_accel.argtypes = [vp, ctypes.POINTER(PoissonAccel)]
+        L.orc_default_poisson_accel.argtypes = [ctypes.POINTER(PoissonAccel)]
+        L.orc_poisson_accel_iterations.argtypes = [vp, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -192,6 +213,27 @@ class Oracle:
 
     def set_poisson(self, check_interval=100, tol=1e-3, iter_cap=0, dual=False):
         self._check(lib().orc_set_poisson(self.h, check_interval, tol, iter_cap, int(dual)))
+
+    def set_poisson_accel(self, variant="split", **kw) -> None:
+        """The reference GPU solver's Poisson accelerators: variant split,
+        fused, onepass or scalar, plus any PoissonAccel field by name
+        (fused_boundary, pressure_relax_scale, fixed_point_relax,
+        scalar_source_scale, source_aware_hh_init, source_aware_hh_scale,
+        anderson_m1, anderson_beta_max, two_grid, two_grid_strength,
+        spatial_diagnostics); unnamed fields take the reference defaults."""
+        a = PoissonAccel()
+        lib().orc_default_poisson_accel(ctypes.byref(a))
+        a.variant = POISSON_VARIANTS[variant]
+        for k, v in kw.items():
+            if k not in dict(PoissonAccel._fields_):
+                raise KeyError(k)
+            setattr(a, k, v)
+        self._check(lib().orc_set_poisson_accel(self.h, ctypes.byref(a)))
+
+    def poisson_accel_iterations(self, k: int) -> None:
+        """k bare iterations of the configured accelerator variant (no
+        projection, gauge or check; fresh Anderson history)."""
+        self._check(lib().orc_poisson_accel_iterations(self.h, k))
 
     def init_two_droplets_sharp(self):
         lib().orc_init_two_droplets_sharp(self.h)
