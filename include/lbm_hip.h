@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define LBM_ABI_VERSION 2
+#define LBM_ABI_VERSION 3
 
 /* Status codes. */
 #define LBM_OK            0
@@ -118,7 +118,18 @@ typedef struct lbm_poisson_check_s {
     int32_t fixed_point_converged;
     int32_t dual_residual_enabled;
     int32_t converged;
+    /* The CSV's last three columns (GPU/InamuroCUDA.cu:2676-2681): the edge
+     * and number of the 4^3 blocks when the check formed block sums (two-grid
+     * correction or spatial diagnostics, lbm_set_poisson_accel), else 0, and
+     * sum_b |block sum of p - p_prev| / sum |p - p_prev| with spatial
+     * diagnostics, else NaN. */
+    int32_t block_size;
+    int32_t block_count;
     int32_t pad;
+    double block_low_frequency_fraction;
+    /* No CSV column: the last Anderson(1) mixing coefficient of the solve so
+     * far, NaN when none was formed. */
+    double anderson_alpha;
 } lbm_poisson_check_t;
 
 typedef void (*lbm_check_fn)(void* user, const lbm_poisson_check_t* check);
@@ -213,6 +224,55 @@ int lbm_upload_dists(void* ctx, const double* f, const double* g, const double* 
  * schedule holds counts that are not multiples of the new check_interval. */
 int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance,
                     uint64_t iteration_limit, int dual_residual);
+
+/* The reference GPU solver's optional Poisson accelerators
+ * (GPU/InamuroCUDA.hpp:60-76, 88; kernels GPU/InamuroCUDA.cu:503-1013,
+ * 1198-1323, loop :1816-2349).  They leave the strict contract
+ * (WHILE_BASELINE.md) and exist for the FP64 path only, the reference's
+ * precision.  One struct replaces the setters:
+ *   variant                setUseFusedPoisson / setUseOnePassPoisson /
+ *                          setUseScalarPoisson (LBM_POISSON_*; the reference
+ *                          gives scalar, then onepass, then fused precedence)
+ *   fused_boundary         setUseFusedBoundaryPressure
+ *   pressure_relax_scale   setPressureRelaxScale (> 0)
+ *   fixed_point_relax      setPoissonFixedPointRelax (> 0; onepass)
+ *   scalar_source_scale    setScalarPoissonSourceScale
+ *   source_aware_hh_init,  setUseSourceAwareHHInit, setSourceAwareHHScale
+ *   source_aware_hh_scale  (not with scalar)
+ *   anderson_m1,           setUsePoissonAndersonM1, setPoissonAndersonBetaMax
+ *   anderson_beta_max      (>= 0; acts with onepass only)
+ *   two_grid,              setUsePoissonTwoGridCorrection,
+ *   two_grid_strength      setPoissonTwoGridStrength (>= 0)
+ *   spatial_diagnostics    setUsePoissonSpatialDiagnostics
+ * Defaults (lbm_poisson_accel_defaults) are the reference's: split, split
+ * boundary, unit relaxations, scalar scale 2, source-aware scale 1, beta 1,
+ * two-grid strength 0.5, all switches off.  The fused variant and the fused
+ * boundary compute the split values bit for bit, so on their own they run
+ * the strict kernel.  As in the reference a step fails with
+ * "fixed-point residual requires frozen split+split Poisson with unit
+ * relaxation and all accelerators disabled" when the dual residual is on
+ * together with an accelerator (spatial diagnostics give the dual-residual
+ * message).  Rejected on mixed-precision contexts; a step with z walls
+ * and an accelerator fails; lbm_adjoint refuses them.  Validation as the
+ * reference's setters; a rejected call changes nothing. */
+enum { LBM_POISSON_SPLIT = 0, LBM_POISSON_FUSED = 1, LBM_POISSON_ONEPASS = 2, LBM_POISSON_SCALAR = 3 };
+typedef struct lbm_poisson_accel_s {
+    int32_t variant;
+    int32_t fused_boundary;
+    double pressure_relax_scale;
+    double fixed_point_relax;
+    double scalar_source_scale;
+    int32_t source_aware_hh_init;
+    int32_t anderson_m1;
+    double source_aware_hh_scale;
+    double anderson_beta_max;
+    double two_grid_strength;
+    int32_t two_grid;
+    int32_t spatial_diagnostics;
+} lbm_poisson_accel_t;
+void lbm_poisson_accel_defaults(lbm_poisson_accel_t* out);
+int lbm_set_poisson_accel(void* ctx, const lbm_poisson_accel_t* settings);
+int lbm_get_poisson_accel(const void* ctx, lbm_poisson_accel_t* out);
 
 /* setUsePoissonGraph (GPU/InamuroCUDA.hpp:80, GPU/InamuroCUDA.cu:2550-2556):
  * replay each check_interval segment of Poisson iterations as a captured HIP

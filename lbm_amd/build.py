@@ -29,6 +29,7 @@ UNITS = [
     ("kernels_f64.hip", ["-ffp-contract=off"]),
     ("kernels_mixed.hip", ["-ffp-contract=fast"]),
     ("kernels_adjoint.hip", ["-ffp-contract=off"]),
+    ("kernels_accel.hip", ["-ffp-contract=off"]),
     ("capi.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ("noalloc.cpp", ["-x", "hip", "-ffp-contract=off"]),
 ]

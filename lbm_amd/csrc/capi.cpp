@@ -127,6 +127,32 @@ struct Ctx {
     int push_cfg = -1;
     std::vector<SegGraph> graphs;
 
+    // The reference's optional Poisson accelerators (FP64; kernels_accel.hip).
+    // Their loop holds h explicitly in h[0], h[1] (and acc_pi for Anderson)
+    // and p in p[pc] / acc_p; AccRoles names which buffer plays which part.
+    lbm_poisson_accel_t accel{};
+    double* acc_p = nullptr;       // second pressure buffer
+    double* acc_pi = nullptr;      // Anderson(1): previous image (15 n)
+    double* acc_pr = nullptr;      // Anderson(1): previous residual (15 n)
+    double* acc_blocks = nullptr;  // 4^3 block sums of p - p_prev
+    struct AccRoles {
+        int h, img, pi;  // 0, 1: h[0], h[1]; 2: acc_pi
+        int pcur;        // 0: p[pc], 1: acc_p
+        bool hist;       // Anderson history stored in this solve
+        bool operator==(const AccRoles& o) const
+        {
+            return h == o.h && img == o.img && pi == o.pi && pcur == o.pcur && hist == o.hist;
+        }
+    };
+    AccRoles acc{};
+    uint64_t acc_iters = 0;  // iterations of the current accelerated solve
+    struct AccGraph {
+        AccRoles start, end;
+        int ci;
+        hipGraphExec_t exec;
+    };
+    std::vector<AccGraph> acc_graphs;
+
     bool timing = false;
     lbm_perf_t perf{};
     hipEvent_t ev[10] = {};
@@ -152,6 +178,7 @@ void free_all(Ctx& c)
     F(c.fei); F(c.rho); F(c.u); F(c.v); F(c.w);
     F(c.tau); F(c.src); F(c.ux); F(c.vy); F(c.wz);
     F(c.p_prev); F(c.partials); F(c.scratch); F(c.scratch1);
+    F(c.acc_p); F(c.acc_pi); F(c.acc_pr); F(c.acc_blocks);
     F(c.dst);
     if (c.hst) (void)hipHostFree(c.hst);
     c.hst = nullptr;
@@ -161,6 +188,8 @@ void free_all(Ctx& c)
     }
     for (auto& gph : c.graphs) (void)hipGraphExecDestroy(gph.exec);
     c.graphs.clear();
+    for (auto& gph : c.acc_graphs) (void)hipGraphExecDestroy(gph.exec);
+    c.acc_graphs.clear();
     if (c.stream) (void)hipStreamDestroy(c.stream);
     c.stream = nullptr;
 }
@@ -297,6 +326,7 @@ Ctx* create(const lbm_params_t& P, int precision)
         HIPCHK(hipHostMalloc(reinterpret_cast<void**>(&c->hst), sizeof(DevState), hipHostMallocDefault));
         for (auto& e : c->ev) HIPCHK(hipEventCreate(&e));
         c->pc = c->pb = 0;
+        lbm_poisson_accel_defaults(&c->accel);
         {
             // Inamuro::initializeArrays, CPU/Inamuro.cpp:519-524.
             std::vector<double> tmp(n, P.rho_G);
@@ -459,6 +489,8 @@ void clear_graphs(Ctx& c)
 {
     for (auto& gph : c.graphs) (void)hipGraphExecDestroy(gph.exec);
     c.graphs.clear();
+    for (auto& gph : c.acc_graphs) (void)hipGraphExecDestroy(gph.exec);
+    c.acc_graphs.clear();
 }
 
 // check_interval iterations, the first in `mode`, the rest H_STREAMED:
@@ -535,6 +567,163 @@ void poisson_check(Ctx& c)
     lbm::f64::check_finalize(s, parts_err, parts_fp, nb, c.dual, c.tol, c.dst);
 }
 
+// ---- the reference's optional Poisson accelerators -------------------------
+
+// Settings whose effect differs from the strict split map: these run the
+// accelerated loop.  The fused variant, the fused boundary kernel, Anderson
+// without onepass and the fixed-point relaxation without onepass compute the
+// split values bit for bit (GPU/InamuroCUDA.cu:2216-2323), so they keep the
+// strict kernel.
+bool accel_on(const Ctx& c)
+{
+    const lbm_poisson_accel_t& a = c.accel;
+    return a.variant == LBM_POISSON_ONEPASS || a.variant == LBM_POISSON_SCALAR ||
+           a.pressure_relax_scale != 1.0 || a.source_aware_hh_init || a.two_grid || a.spatial_diagnostics;
+}
+
+// Settings the reference's fixed-point residual refuses
+// (GPU/InamuroCUDA.cu:1846-1857), effective or not.
+bool accel_requested(const Ctx& c)
+{
+    const lbm_poisson_accel_t& a = c.accel;
+    return a.variant != LBM_POISSON_SPLIT || a.fused_boundary || a.source_aware_hh_init || a.anderson_m1 ||
+           a.two_grid || a.pressure_relax_scale != 1.0 || a.fixed_point_relax != 1.0;
+}
+
+double* acc_h(Ctx& c, int role) { return role == 2 ? c.acc_pi : D<double>(c.h[role]); }
+double* acc_pbuf(Ctx& c, int which) { return which == 0 ? c.p[c.pc] : c.acc_p; }
+bool acc_anderson(const Ctx& c) { return c.accel.anderson_m1 && c.accel.variant == LBM_POISSON_ONEPASS; }
+
+// Start of an accelerated solve: buffers, the explicit h the loop iterates
+// (update_hh's E p, or the uploaded post-boundary h), and the source-aware
+// start (sourceAwareHHInitKernel, GPU/InamuroCUDA.cu:2162-2172).
+void accel_begin(Ctx& c)
+{
+    const lbm_poisson_accel_t& a = c.accel;
+    const size_t n = c.n();
+    const bool scalar = a.variant == LBM_POISSON_SCALAR;
+    if (!c.acc_p) c.acc_p = dalloc(n);
+    int nbx = 0, nby = 0;
+    if (!c.acc_blocks) c.acc_blocks = dalloc(static_cast<size_t>(lbm::acc::block_count(c.g, &nbx, &nby)));
+    if (acc_anderson(c) && !c.acc_pi) {
+        c.acc_pi = dalloc(lbm::Q * n);
+        c.acc_pr = dalloc(lbm::Q * n);
+    }
+    Ctx::AccRoles r{};
+    if (c.hs == HState::Equilibrium) {
+        r.h = 0;
+        r.img = 1;
+        if (!scalar) lbm::f64::equilibrium_h(c.stream, c.g, c.p[c.pc], D<double>(c.h[0]));
+    } else {
+        r.h = c.hcur;
+        r.img = 1 - c.hcur;
+    }
+    r.pi = 2;
+    r.pcur = 0;
+    r.hist = false;
+    if (a.source_aware_hh_init && !scalar)
+        lbm::acc::source_init(c.stream, c.g, a.source_aware_hh_scale, c.p[c.pc], D<double>(c.tau),
+                              D<double>(c.src), acc_h(c, r.h));
+    HIPCHK(hipGetLastError());
+    c.acc = r;
+    c.acc_iters = 0;
+}
+
+// One accelerated iteration on the buffers named by r (GPU/InamuroCUDA.cu:
+// 2216-2294); r advances to the roles after it.
+void accel_iteration(Ctx& c, Ctx::AccRoles& r)
+{
+    const lbm_poisson_accel_t& a = c.accel;
+    hipStream_t s = c.stream;
+    const Grid& g = c.g;
+    double* p = acc_pbuf(c, r.pcur);
+    double* pn = acc_pbuf(c, 1 - r.pcur);
+    if (a.variant == LBM_POISSON_SCALAR) {
+        lbm::acc::scalar(s, g, a.scalar_source_scale, p, D<double>(c.rho), D<double>(c.src), pn);
+        r.pcur = 1 - r.pcur;
+        return;
+    }
+    const bool onepass = a.variant == LBM_POISSON_ONEPASS;
+    double* h = acc_h(c, r.h);
+    double* img = acc_h(c, r.img);
+    const double* tau = D<double>(c.tau);
+    const double* src = D<double>(c.src);
+    if (!acc_anderson(c) || !r.hist) {
+        const int mode = acc_anderson(c) ? 1 : 0;
+        lbm::acc::pull(s, g, onepass, mode, a.pressure_relax_scale, a.fixed_point_relax, h, p, tau, src, img,
+                       pn, c.acc_pr, mode ? acc_h(c, r.pi) : nullptr, nullptr);
+        std::swap(r.h, r.img);
+        r.pcur = 1 - r.pcur;
+        if (mode) r.hist = true;
+        return;
+    }
+    lbm::acc::pull(s, g, 1, 2, a.pressure_relax_scale, a.fixed_point_relax, h, p, tau, src, img, pn,
+                   c.acc_pr, nullptr, c.partials);
+    lbm::acc::anderson_mix(s, g, a.anderson_beta_max, c.partials, c.dst, h, img, acc_h(c, r.pi), c.acc_pr, pn);
+    // The mixed state went into the previous image's buffer: it is the new h,
+    // the image becomes the previous image, the old h is free for the next one.
+    const int old_h = r.h;
+    r.h = r.pi;
+    r.pi = r.img;
+    r.img = old_h;
+    r.pcur = 1 - r.pcur;
+}
+
+// check_interval accelerated iterations, eagerly or as a captured graph per
+// starting role assignment (the kernels' arguments, settings included, are
+// fixed by it; lbm_set_poisson_accel drops the cache).
+void accel_segment(Ctx& c)
+{
+    c.acc_iters += static_cast<uint64_t>(c.check_interval);
+    if (!c.use_graphs) {
+        for (int k = 0; k < c.check_interval; ++k) accel_iteration(c, c.acc);
+        return;
+    }
+    for (const auto& gph : c.acc_graphs) {
+        if (gph.start == c.acc && gph.ci == c.check_interval) {
+            HIPCHK(hipGraphLaunch(gph.exec, c.stream));
+            c.acc = gph.end;
+            return;
+        }
+    }
+    Ctx::AccGraph gph{c.acc, c.acc, c.check_interval, nullptr};
+    HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    hipGraph_t graph = nullptr;
+    try {
+        for (int k = 0; k < c.check_interval; ++k) accel_iteration(c, gph.end);
+        HIPCHK(hipGetLastError());
+    } catch (...) {
+        (void)hipStreamEndCapture(c.stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(c.stream, &graph));
+    const hipError_t e = hipGraphInstantiate(&gph.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) throw HipError{e, "hipGraphInstantiate"};
+    c.acc_graphs.push_back(gph);
+    HIPCHK(hipGraphLaunch(gph.exec, c.stream));
+    c.acc = gph.end;
+}
+
+// check_residual of the accelerated loop (GPU/InamuroCUDA.cu:1960-2160).
+void accel_check(Ctx& c)
+{
+    const lbm_poisson_accel_t& a = c.accel;
+    const bool scalar = a.variant == LBM_POISSON_SCALAR;
+    lbm::acc::check(c.stream, c.g, c.ph, c.tol, a.two_grid || a.spatial_diagnostics, a.spatial_diagnostics,
+                    a.two_grid, a.two_grid_strength, acc_pbuf(c, c.acc.pcur), c.p_prev,
+                    scalar ? nullptr : acc_h(c, c.acc.h), c.acc_blocks, c.partials, c.dst);
+}
+
+// The solved pressure back into p[pc], where correct_uvw and the next step
+// read it.
+void accel_end(Ctx& c)
+{
+    if (c.acc.pcur == 1)
+        HIPCHK(hipMemcpyAsync(c.p[c.pc], c.acc_p, c.n() * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+}
+
 // Called after every completed convergence check with the iteration count
 // (the adjoint's replays record raw-h snapshots through it).
 struct CheckHook {
@@ -546,6 +735,19 @@ struct CheckHook {
 void step_one(Ctx& c, const CheckHook* hook = nullptr)
 {
     hipStream_t s = c.stream;
+    const bool acc = accel_on(c);
+    // doPressurePoisson's refusals (GPU/InamuroCUDA.cu:1846-1865), made
+    // before the step starts so that a refused call changes nothing.
+    if (c.dual != 0 && accel_requested(c))
+        throw ApiError{LBM_ERR_STATE,
+                       "fixed-point residual requires frozen split+split Poisson with unit relaxation and all "
+                       "accelerators disabled"};
+    if (c.dual != 0 && c.accel.spatial_diagnostics)
+        throw ApiError{LBM_ERR_STATE,
+                       "dual-residual exit requires check_interval=100, tolerance=1e-3, poisson_iteration_limit=0, "
+                       "deterministic reductions, no spatial diagnostics"};
+    if (acc && c.g.zwall)
+        throw ApiError{LBM_ERR_STATE, "Poisson accelerators support the reference's periodic z only (no z walls)"};
     step_fg(c);
 
     ++c.solve_calls;
@@ -562,7 +764,10 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
 
     int hmode = c.hs == HState::Equilibrium ? lbm::H_EQUILIBRIUM : lbm::H_EXPLICIT;
     const uint64_t forced = c.sched_pos < c.schedule.size() ? c.schedule[c.sched_pos] : 0;
-    choose_push_config(c);
+    if (acc)
+        accel_begin(c);
+    else
+        choose_push_config(c);
     uint64_t iter = 0;
     bool first_segment = true;
     for (;;) {
@@ -570,10 +775,16 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
         if (iter > std::numeric_limits<uint64_t>::max() - static_cast<uint64_t>(c.check_interval))
             throw ApiError{LBM_ERR_OVERFLOW, "Poisson iteration counter overflow"};
         iter += static_cast<uint64_t>(c.check_interval);
-        poisson_segment(c, hmode);
+        if (acc)
+            accel_segment(c);
+        else
+            poisson_segment(c, hmode);
         HIPCHK(hipGetLastError());
         record(c, 5);
-        poisson_check(c);
+        if (acc)
+            accel_check(c);
+        else
+            poisson_check(c);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(c.hst, c.dst, sizeof(DevState), hipMemcpyDeviceToHost, s));
         record(c, 6);
@@ -635,6 +846,17 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
             rec.fixed_point_converged = d.fixed_point_converged;
             rec.dual_residual_enabled = d.dual_residual_enabled;
             rec.converged = d.converged;
+            rec.block_low_frequency_fraction = std::numeric_limits<double>::quiet_NaN();
+            rec.anderson_alpha = std::numeric_limits<double>::quiet_NaN();
+            if (acc && (c.accel.two_grid || c.accel.spatial_diagnostics)) {
+                int nbx = 0, nby = 0;
+                rec.block_count = lbm::acc::block_count(c.g, &nbx, &nby);
+                rec.block_size = 4;
+                // GPU/InamuroCUDA.cu:2107-2110
+                if (c.accel.spatial_diagnostics)
+                    rec.block_low_frequency_fraction = st.e1 > 0.0 ? st.acc_block_abs / st.e1 : 0.0;
+            }
+            if (acc && acc_anderson(c) && c.acc_iters >= 2) rec.anderson_alpha = st.acc_alpha;
             c.check_observer(c.check_user, &rec);
         }
         if (hook) hook->fn(c, iter, hook->arg);
@@ -651,6 +873,7 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
         }
         hmode = lbm::H_GAUGED;
     }
+    if (acc) accel_end(c);
     c.last_iterations = iter;
     c.perf.poisson_iterations += iter;
     c.history.push_back(iter);
@@ -1637,6 +1860,58 @@ int lbm_set_poisson(void* ctx, int32_t check_interval, double tolerance, uint64_
     });
 }
 
+void lbm_poisson_accel_defaults(lbm_poisson_accel_t* out)
+{
+    if (!out) return;
+    *out = lbm_poisson_accel_t{};
+    // RunConfig / InamuroCUDA defaults, GPU/main.cpp:27-35, GPU/InamuroCUDA.hpp:152-157.
+    out->variant = LBM_POISSON_SPLIT;
+    out->pressure_relax_scale = 1.0;
+    out->fixed_point_relax = 1.0;
+    out->scalar_source_scale = 2.0;
+    out->source_aware_hh_scale = 1.0;
+    out->anderson_beta_max = 1.0;
+    out->two_grid_strength = 0.5;
+}
+
+int lbm_set_poisson_accel(void* ctx, const lbm_poisson_accel_t* settings)
+{
+    return guarded(ctx, [&](Ctx& c) {
+        if (!settings) throw ApiError{LBM_ERR_ARG, "settings must not be NULL"};
+        const lbm_poisson_accel_t& a = *settings;
+        // GPU/main.cpp:205-227 and the setters, GPU/InamuroCUDA.cu:2488-2540.
+        if (a.variant < LBM_POISSON_SPLIT || a.variant > LBM_POISSON_SCALAR)
+            throw ApiError{LBM_ERR_ARG, "--poisson must be split, fused, onepass, or scalar"};
+        if (!(a.pressure_relax_scale > 0.0))
+            throw ApiError{LBM_ERR_ARG, "pressure relaxation scale must be positive"};
+        if (!(a.fixed_point_relax > 0.0))
+            throw ApiError{LBM_ERR_ARG, "Poisson fixed-point relaxation must be positive"};
+        if (!(a.anderson_beta_max >= 0.0))
+            throw ApiError{LBM_ERR_ARG, "Poisson Anderson beta max must be non-negative"};
+        if (!(a.two_grid_strength >= 0.0))
+            throw ApiError{LBM_ERR_ARG, "Poisson two-grid strength must be non-negative"};
+        if (!std::isfinite(a.scalar_source_scale) || !std::isfinite(a.source_aware_hh_scale) ||
+            !std::isfinite(a.pressure_relax_scale) || !std::isfinite(a.fixed_point_relax) ||
+            !std::isfinite(a.anderson_beta_max) || !std::isfinite(a.two_grid_strength))
+            throw ApiError{LBM_ERR_ARG, "Poisson accelerator settings must be finite"};
+        lbm_poisson_accel_t def{};
+        lbm_poisson_accel_defaults(&def);
+        const bool nondefault = std::memcmp(&a, &def, sizeof(a)) != 0;
+        if (c.mixed && nondefault)
+            throw ApiError{LBM_ERR_ARG,
+                           "the Poisson accelerators are implemented for the FP64 path (the reference's precision)"};
+        c.accel = a;
+        clear_graphs(c);
+    });
+}
+
+int lbm_get_poisson_accel(const void* ctx, lbm_poisson_accel_t* out)
+{
+    if (!ctx || !out) return LBM_ERR_ARG;
+    *out = static_cast<const Ctx*>(ctx)->accel;
+    return LBM_OK;
+}
+
 int lbm_set_z_walls(void* ctx, int enabled)
 {
     return guarded(ctx, [&](Ctx& c) {
@@ -1826,6 +2101,9 @@ int lbm_adjoint(void* ctx, const double* phi0, const double* target, int32_t ste
                 double* grad_phi0, double* grad_kg)
 {
     return guarded(ctx, [&](Ctx& c) {
+        if (accel_on(c) || accel_requested(c))
+            throw ApiError{LBM_ERR_STATE,
+                           "the adjoint differentiates the strict split Poisson map; turn the Poisson accelerators off"};
         run_adjoint(c, phi0, target, steps, checkpoint_interval, iterations, objective, grad_phi0, grad_kg);
     });
 }

@@ -48,6 +48,8 @@ struct DevState {
     double fp[6];          // dual-residual sums
     double fp_h_rel, fp_p_rel, fp_rel;
     double mass;           // scratch for lbm_mass
+    double acc_alpha;      // Poisson accelerators: last Anderson(1) coefficient
+    double acc_block_abs;  // Poisson accelerators: sum_b |4^3 block sum of p - p_prev|
     int32_t status;
     int32_t converged;
     int32_t pressure_converged;
@@ -183,6 +185,38 @@ void stencil(hipStream_t s, const Grid& g, const double* pc, double* lfei, doubl
 constexpr int PC_SLOTS = 24;  // per-cell adjoint slots of collide -> stencil
 
 } // namespace adj
+
+// The reference GPU solver's optional Poisson accelerators, FP64
+// (kernels_accel.hip).  h is explicit and post-boundary, p is written by
+// every iteration.
+namespace acc {
+
+// One pull-form iteration h -> hn, p -> pn with relaxation scale `scale`;
+// onepass adds the fixed-point relaxation omega.  anderson (onepass only):
+// 0 none, 1 first iteration (also writes the history pr = hn - h, pi = hn),
+// 2 later iterations (block partials of the Anderson dot products into
+// parts, 2 * grid_blocks(g.n) doubles; pn is left to anderson_mix).
+void pull(hipStream_t s, const Grid& g, int onepass, int anderson, double scale, double omega,
+          const double* h, const double* p, const double* tau, const double* src, double* hn,
+          double* pn, double* pr, double* pi, double* parts);
+// Anderson(1) coefficient from the partials (into st->acc_alpha), then the
+// mixed state into pi_to_a (which becomes h), pr = img - h, pn = sum_q.
+void anderson_mix(hipStream_t s, const Grid& g, double beta_max, const double* parts, DevState* st,
+                  const double* h, const double* img, double* pi_to_a, double* pr, double* pn);
+void scalar(hipStream_t s, const Grid& g, double scale, const double* p, const double* rho,
+            const double* src, double* pn);
+void source_init(hipStream_t s, const Grid& g, double scale, const double* p, const double* tau,
+                 const double* src, double* h);
+// Number of 4^3 blocks of the grid (and blocks along x and y).
+int block_count(const Grid& g, int* nbx, int* nby);
+// A convergence check on explicit p (and h unless nullptr): gauge, relative
+// change (with 4^3 block sums into `blocks` when blocks_needed), convergence,
+// then the spatial sum_b |B_b| and the two-grid correction when asked.
+void check(hipStream_t s, const Grid& g, const Phys& ph, double tol, int blocks_needed,
+           int spatial, int two_grid, double strength, double* p, double* p_prev, double* h,
+           double* blocks, double* partials, DevState* st);
+
+} // namespace acc
 
 // Message of the last failed call made without a context (lbm_create,
 // lbm_mass_noalloc), returned by lbm_last_error(NULL).

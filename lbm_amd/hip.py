@@ -46,6 +46,7 @@ EXPORTED = (
     "lbm_set_z_walls", "lbm_set_gravity", "lbm_set_adjoint_objective",
     "lbm_set_adjoint_state_precision", "lbm_set_progress", "lbm_set_check_observer",
     "lbm_mass_noalloc", "lbm_set_poisson_launch", "lbm_get_poisson_launch",
+    "lbm_poisson_accel_defaults", "lbm_set_poisson_accel", "lbm_get_poisson_accel",
 )
 
 # lbm_progress_fn: (user, phase, step, iterations)
@@ -96,7 +97,25 @@ class PoissonCheck(ctypes.Structure):
         ("fixed_point_p_relative", ctypes.c_double), ("fixed_point_relative", ctypes.c_double),
         ("fixed_point_evaluated", ctypes.c_int32), ("pressure_converged", ctypes.c_int32),
         ("fixed_point_converged", ctypes.c_int32), ("dual_residual_enabled", ctypes.c_int32),
-        ("converged", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("converged", ctypes.c_int32), ("block_size", ctypes.c_int32),
+        ("block_count", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("block_low_frequency_fraction", ctypes.c_double), ("anderson_alpha", ctypes.c_double),
+    ]
+
+
+POISSON_VARIANTS = {"split": 0, "fused": 1, "onepass": 2, "scalar": 3}
+
+
+class PoissonAccel(ctypes.Structure):
+    """lbm_poisson_accel_t: the reference GPU solver's Poisson accelerator
+    settings (GPU/InamuroCUDA.hpp:60-76, 88)."""
+    _fields_ = [
+        ("variant", ctypes.c_int32), ("fused_boundary", ctypes.c_int32),
+        ("pressure_relax_scale", ctypes.c_double), ("fixed_point_relax", ctypes.c_double),
+        ("scalar_source_scale", ctypes.c_double), ("source_aware_hh_init", ctypes.c_int32),
+        ("anderson_m1", ctypes.c_int32), ("source_aware_hh_scale", ctypes.c_double),
+        ("anderson_beta_max", ctypes.c_double), ("two_grid_strength", ctypes.c_double),
+        ("two_grid", ctypes.c_int32), ("spatial_diagnostics", ctypes.c_int32),
     ]
 
 
@@ -174,6 +193,10 @@ def load_library():
     L.lbm_mass_noalloc.argtypes = [vp, ctypes.c_int64, vp, vp]
     L.lbm_set_poisson_launch.argtypes = [vp, ctypes.c_int]
     L.lbm_get_poisson_launch.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+    L.lbm_poisson_accel_defaults.restype = None
+    L.lbm_poisson_accel_defaults.argtypes = [ctypes.POINTER(PoissonAccel)]
+    L.lbm_set_poisson_accel.argtypes = [vp, ctypes.POINTER(PoissonAccel)]
+    L.lbm_get_poisson_accel.argtypes = [vp, ctypes.POINTER(PoissonAccel)]
     _lib = L
     return L
 
@@ -346,6 +369,78 @@ class InamuroHIP:
     def set_gravity(self, gz: float):
         """Gravitational acceleration along z (extension; 0 by default)."""
         self._check(load_library().lbm_set_gravity(self._h, float(gz)))
+
+    # -- the reference's Poisson accelerators (FP64) --------------------------
+    def poisson_accel(self) -> dict:
+        a = PoissonAccel()
+        self._check(load_library().lbm_get_poisson_accel(self._h, ctypes.byref(a)))
+        return {k: getattr(a, k) for k, _ in PoissonAccel._fields_}
+
+    def set_poisson_accel(self, variant=None, **kw):
+        """Changes the named accelerator settings (lbm_poisson_accel_t fields;
+        variant 'split', 'fused', 'onepass' or 'scalar'), keeping the others."""
+        a = PoissonAccel()
+        self._check(load_library().lbm_get_poisson_accel(self._h, ctypes.byref(a)))
+        if variant is not None:
+            a.variant = POISSON_VARIANTS[variant] if isinstance(variant, str) else int(variant)
+        names = dict(PoissonAccel._fields_)
+        for k, v in kw.items():
+            if k not in names:
+                raise KeyError(k)
+            setattr(a, k, v)
+        self._check(load_library().lbm_set_poisson_accel(self._h, ctypes.byref(a)))
+
+    def _variant_flag(self, name: str, enabled: bool):
+        # setUseFusedPoisson / setUseOnePassPoisson / setUseScalarPoisson keep
+        # three flags; doPressurePoisson gives scalar, then onepass, then fused
+        # precedence (GPU/InamuroCUDA.cu:2216-2296).
+        flags = getattr(self, "_variant_flags", {"fused": False, "onepass": False, "scalar": False})
+        flags = dict(flags, **{name: bool(enabled)})
+        variant = next((v for v in ("scalar", "onepass", "fused") if flags[v]), "split")
+        self.set_poisson_accel(variant=variant)
+        self._variant_flags = flags
+
+    def setUseFusedPoisson(self, enabled: bool):
+        self._variant_flag("fused", enabled)
+
+    def setUseOnePassPoisson(self, enabled: bool):
+        self._variant_flag("onepass", enabled)
+
+    def setUseScalarPoisson(self, enabled: bool):
+        self._variant_flag("scalar", enabled)
+
+    def setScalarPoissonSourceScale(self, scale: float):
+        self.set_poisson_accel(scalar_source_scale=float(scale))
+
+    def setUseSourceAwareHHInit(self, enabled: bool):
+        self.set_poisson_accel(source_aware_hh_init=int(bool(enabled)))
+
+    def setSourceAwareHHScale(self, scale: float):
+        self.set_poisson_accel(source_aware_hh_scale=float(scale))
+
+    def setPressureRelaxScale(self, scale: float):
+        self.set_poisson_accel(pressure_relax_scale=float(scale))
+
+    def setPoissonFixedPointRelax(self, omega: float):
+        self.set_poisson_accel(fixed_point_relax=float(omega))
+
+    def setUsePoissonAndersonM1(self, enabled: bool):
+        self.set_poisson_accel(anderson_m1=int(bool(enabled)))
+
+    def setPoissonAndersonBetaMax(self, value: float):
+        self.set_poisson_accel(anderson_beta_max=float(value))
+
+    def setUsePoissonTwoGridCorrection(self, enabled: bool):
+        self.set_poisson_accel(two_grid=int(bool(enabled)))
+
+    def setPoissonTwoGridStrength(self, value: float):
+        self.set_poisson_accel(two_grid_strength=float(value))
+
+    def setUseFusedBoundaryPressure(self, enabled: bool):
+        self.set_poisson_accel(fused_boundary=int(bool(enabled)))
+
+    def setUsePoissonSpatialDiagnostics(self, enabled: bool):
+        self.set_poisson_accel(spatial_diagnostics=int(bool(enabled)))
 
     def setUsePoissonGraph(self, enabled: bool):
         """Replay each check-interval segment as a captured HIP graph (default on)."""

@@ -36,15 +36,30 @@ def test_struct_sizes_match_header(tmp_path):
     # compile a tiny C program against the header and compare sizeof()
     src = str(tmp_path / "sizes_probe.c")
     with open(src, "w") as fh:
-        fh.write('#include <stdio.h>\n#include "lbm_hip.h"\nint main(void){printf("%zu %zu %zu %zu %zu\\n",'
+        fh.write('#include <stdio.h>\n#include "lbm_hip.h"\nint main(void){printf("%zu %zu %zu %zu %zu %zu\\n",'
                  ' sizeof(lbm_params_t), sizeof(lbm_poisson_diag_t), sizeof(lbm_perf_t),'
-                 ' sizeof(lbm_adjoint_stats_t), sizeof(lbm_poisson_check_t)); return 0;}\n')
+                 ' sizeof(lbm_adjoint_stats_t), sizeof(lbm_poisson_check_t),'
+                 ' sizeof(lbm_poisson_accel_t)); return 0;}\n')
     exe = src[:-2]
     subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), src, "-o", exe], check=True)
     sizes = list(map(int, subprocess.run([exe], capture_output=True, text=True).stdout.split()))
     assert sizes == [ctypes.sizeof(hip.LbmParams), ctypes.sizeof(hip.PoissonStepDiagnostics),
                      ctypes.sizeof(hip.PerformanceMetrics), ctypes.sizeof(hip.AdjointStats),
-                     ctypes.sizeof(hip.PoissonCheck)]
+                     ctypes.sizeof(hip.PoissonCheck), ctypes.sizeof(hip.PoissonAccel)]
+
+
+def test_poisson_accelerator_defaults_are_the_reference_defaults():
+    """RunConfig (GPU/main.cpp:27-35) and InamuroCUDA (GPU/InamuroCUDA.hpp:
+    136-157): split, split boundary, unit relaxations, scalar source scale 2,
+    source-aware scale 1, Anderson beta 1, two-grid strength 0.5, all off."""
+    a = hip.PoissonAccel()
+    hip.load_library().lbm_poisson_accel_defaults(ctypes.byref(a))
+    got = {k: getattr(a, k) for k, _ in hip.PoissonAccel._fields_}
+    assert got == {"variant": 0, "fused_boundary": 0, "pressure_relax_scale": 1.0,
+                   "fixed_point_relax": 1.0, "scalar_source_scale": 2.0, "source_aware_hh_init": 0,
+                   "anderson_m1": 0, "source_aware_hh_scale": 1.0, "anderson_beta_max": 1.0,
+                   "two_grid_strength": 0.5, "two_grid": 0, "spatial_diagnostics": 0}
+    assert hip.load_library().lbm_set_poisson_accel(None, ctypes.byref(a)) == hip.LBM_ERR_ARG
 
 
 def test_null_arguments_are_rejected_without_touching_the_device():
