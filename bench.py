@@ -23,7 +23,9 @@ HIP events on the library's stream.  `cpu_baseline` times the CPU oracle
 Other legs: --workload head_on_collision | falling_droplet | static_droplet_64
 (...); --adjoint-only [--adjoint-steps T --adjoint-checkpoint K --adjoint-fd N];
 --adjoint-falling T (the falling droplet's centroid-height adjoint at full
-size); --traffic-probe (one step per precision, for counter runs).
+size); --traffic-probe (one step per precision, for counter runs); --accel
+(the reference's Poisson accelerators: iterations to tolerance and time per
+iteration).
 """
 from __future__ import annotations
 
@@ -335,6 +337,86 @@ def run_adjoint_falling(hip, steps, checkpoint, state, fd=False, h=1e-6, iterati
     return rec
 
 
+# The reference's optional Poisson accelerators (FP64), each as lbm_gpu would
+# set it, and the algorithmic bytes per cell of one of its iterations: the
+# pull/onepass map moves what the strict iteration does (h read and written,
+# p read and written, tau_h and the source read); the scalar Jacobi sweep
+# moves p, rho and the source; an Anderson(1) iteration also reads the
+# previous residual in the map (384) and reads three / writes two
+# distributions plus p to mix (608).
+ACCEL_VARIANTS = [
+    ("strict split", {}, 272),
+    ("fused + fused boundary", dict(variant="fused", fused_boundary=1), 272),
+    ("split, relax scale 0.85", dict(pressure_relax_scale=0.85), 272),
+    ("onepass", dict(variant="onepass"), 272),
+    ("onepass + Anderson(1)", dict(variant="onepass", anderson_m1=1), 992),
+    ("scalar Jacobi", dict(variant="scalar"), 32),
+    ("split + two-grid 0.5", dict(two_grid=1), 272),
+    ("split + source-aware start", dict(source_aware_hh_init=1), 272),
+    ("onepass + Anderson + two-grid + source-aware",
+     dict(variant="onepass", anderson_m1=1, two_grid=1, source_aware_hh_init=1), 992),
+]
+
+
+def run_accel(hip, steps=3, bw_iterations=1000):
+    """Convergence and bandwidth of each accelerator.  Convergence: the 64^3
+    static droplet (FP64), `steps` steps from the cold start, iterations per
+    step, Poisson time, and the fields' distance from the strict run's.
+    Bandwidth: the 256^3 droplet with the iteration counts frozen at
+    [100, bw_iterations], time per iteration of the second solve."""
+    out = []
+    strict = None
+    for name, accel, bpc in ACCEL_VARIANTS:
+        rec = {"variant": name, "settings": accel, "bytes_per_cell": bpc}
+        sim, P = make_sim(hip, WORKLOADS["static_droplet_64"], "fp64")
+        sim.set_poisson_accel(**accel)
+        sim.set_timing(True)
+        t0 = time.perf_counter()
+        try:
+            its = sim.step(steps)
+        except RuntimeError as e:
+            rec["error"] = str(e)
+            sim.close()
+            out.append(rec)
+            continue
+        sim.synchronize()
+        pf = sim.perf()
+        f = sim.downloadFieldsToCPU()
+        sim.close()
+        rec.update(static_droplet_64_iterations=its, static_droplet_64_poisson_ms=pf.poisson_ms,
+                   static_droplet_64_wall_ms=(time.perf_counter() - t0) * 1e3)
+        if strict is None:
+            strict = f
+        else:
+            err, _ = field_errors(f, strict)
+            rec["relative_linf_vs_strict"] = {k: err[k]["linf_rel"] for k in ("p", "u", "fei")}
+        print(f"accel {name}: {its} iterations, Poisson {pf.poisson_ms:.1f} ms", file=sys.stderr, flush=True)
+
+        sim, P = make_sim(hip, WORKLOADS["static_droplet_256"], "fp64")
+        sim.set_poisson_accel(**accel)
+        sim.set_iteration_schedule([100, bw_iterations])
+        try:
+            sim.performTimeStepGPU(1)
+            sim.set_timing(True)
+            sim.reset_perf()
+            sim.performTimeStepGPU(1)
+            sim.synchronize()
+        except RuntimeError as e:
+            rec["error_256"] = str(e)
+            sim.close()
+            out.append(rec)
+            continue
+        pf = sim.perf()
+        sim.close()
+        cells = P.lx * P.ly * P.lz
+        it_ms = pf.poisson_ms / max(pf.poisson_iter_launches, 1)
+        gbs = bpc * cells / (it_ms * 1e-3) / 1e9
+        rec.update(static_droplet_256_ms_per_iteration=it_ms, achieved_gbs=gbs, frac=gbs / HBM_PEAK_GBS)
+        print(f"accel {name}: 256^3 {it_ms:.3f} ms/iteration, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+        out.append(rec)
+    return {"accelerators": out}
+
+
 def copy_ceiling():
     """HBM bandwidths (GB/s) that copies reach on this GPU, by access
     pattern, from lbm_amd/copy_probe's ceiling mode run as a child process;
@@ -437,6 +519,9 @@ def main(argv=None):
     ap.add_argument("--adjoint-falling-state", choices=["fp32", "fp64"], default="fp32")
     ap.add_argument("--adjoint-falling-fd", action="store_true",
                     help="with --adjoint-falling: central differences in k_g")
+    ap.add_argument("--accel", action="store_true",
+                    help="the reference's Poisson accelerators (FP64): iterations to tolerance on the 64^3 "
+                         "droplet and time per iteration at 256^3; prints one JSON line and exits")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one short solve per precision (for rocprofv3 --pmc runs) and exit")
     ap.add_argument("--adjoint-probe", action="store_true",
@@ -453,6 +538,9 @@ def main(argv=None):
     global GRAPHS
     GRAPHS = not args.no_graphs
 
+    if args.accel:
+        print(json.dumps(run_accel(hip)), flush=True)
+        return 0
     if args.adjoint_probe:
         rec = run_adjoint_falling(hip, 2, 2, "fp32", iterations=[100, 300])
         print(json.dumps(rec), flush=True)

@@ -40,6 +40,7 @@ InamuroHIP::InamuroHIP(const lbm_params_t& params, int precision)
 {
     check(lbm_create(&params, precision, &ctx), nullptr);
     n = static_cast<std::size_t>(params.lx) * params.ly * params.lz;
+    lbm_poisson_accel_defaults(&accel_);
 }
 
 InamuroHIP::~InamuroHIP()
@@ -96,11 +97,72 @@ void InamuroHIP::setUsePoissonGraph(bool enabled)
     check(lbm_set_poisson_graph(ctx, enabled ? 1 : 0), ctx);
 }
 
+// The whole accelerator set goes through the library, which validates it
+// with the reference setters' messages; only an accepted set is kept.
+void InamuroHIP::applyAccel(const lbm_poisson_accel_t& next)
+{
+    check(lbm_set_poisson_accel(ctx, &next), ctx);
+    accel_ = next;
+}
+
+namespace {
+// doPressurePoisson tests scalar, then onepass, then fused
+// (GPU/InamuroCUDA.cu:2216-2296).
+int variant_of(bool fused, bool onepass, bool scalar)
+{
+    return scalar ? LBM_POISSON_SCALAR : (onepass ? LBM_POISSON_ONEPASS : (fused ? LBM_POISSON_FUSED : LBM_POISSON_SPLIT));
+}
+} // namespace
+
+void InamuroHIP::setUseFusedPoisson(bool enabled)
+{
+    lbm_poisson_accel_t a = accel_;
+    a.variant = variant_of(enabled, onepass_, scalar_);
+    applyAccel(a);
+    fused_ = enabled;
+}
+
+void InamuroHIP::setUseOnePassPoisson(bool enabled)
+{
+    lbm_poisson_accel_t a = accel_;
+    a.variant = variant_of(fused_, enabled, scalar_);
+    applyAccel(a);
+    onepass_ = enabled;
+}
+
+void InamuroHIP::setUseScalarPoisson(bool enabled)
+{
+    lbm_poisson_accel_t a = accel_;
+    a.variant = variant_of(fused_, onepass_, enabled);
+    applyAccel(a);
+    scalar_ = enabled;
+}
+
+#define LBM_ACCEL_SETTER(name, type, field, conv) \
+    void InamuroHIP::name(type value)             \
+    {                                             \
+        lbm_poisson_accel_t a = accel_;           \
+        a.field = conv;                           \
+        applyAccel(a);                            \
+    }
+LBM_ACCEL_SETTER(setScalarPoissonSourceScale, double, scalar_source_scale, value)
+LBM_ACCEL_SETTER(setUseSourceAwareHHInit, bool, source_aware_hh_init, value ? 1 : 0)
+LBM_ACCEL_SETTER(setSourceAwareHHScale, double, source_aware_hh_scale, value)
+LBM_ACCEL_SETTER(setPressureRelaxScale, double, pressure_relax_scale, value)
+LBM_ACCEL_SETTER(setPoissonFixedPointRelax, double, fixed_point_relax, value)
+LBM_ACCEL_SETTER(setUsePoissonAndersonM1, bool, anderson_m1, value ? 1 : 0)
+LBM_ACCEL_SETTER(setPoissonAndersonBetaMax, double, anderson_beta_max, value)
+LBM_ACCEL_SETTER(setUsePoissonTwoGridCorrection, bool, two_grid, value ? 1 : 0)
+LBM_ACCEL_SETTER(setPoissonTwoGridStrength, double, two_grid_strength, value)
+LBM_ACCEL_SETTER(setUseFusedBoundaryPressure, bool, fused_boundary, value ? 1 : 0)
+LBM_ACCEL_SETTER(setUsePoissonSpatialDiagnostics, bool, spatial_diagnostics, value ? 1 : 0)
+#undef LBM_ACCEL_SETTER
+
 namespace {
 
-// One row of writePoissonDiagnostic (GPU/InamuroCUDA.cu:2652-2681) with
-// spatial diagnostics off: empty block_low_frequency_fraction, block size
-// and count 0.
+// One row of writePoissonDiagnostic (GPU/InamuroCUDA.cu:2652-2681): the
+// block columns are empty / 0 / 0 unless the check formed 4^3 block sums
+// (two-grid correction or spatial diagnostics).
 void write_check_row(void* user, const lbm_poisson_check_t* r)
 {
     std::ofstream& out = *static_cast<std::ofstream*>(user);
@@ -120,7 +182,8 @@ void write_check_row(void* user, const lbm_poisson_check_t* r)
     out << (r->fixed_point_evaluated ? 1 : 0) << ',' << (r->pressure_converged ? 1 : 0) << ','
         << (r->fixed_point_converged ? 1 : 0) << ',' << (r->dual_residual_enabled ? 1 : 0) << ','
         << (r->converged ? 1 : 0) << ',';
-    out << ',' << 0 << ',' << 0 << '\n';
+    if (std::isfinite(r->block_low_frequency_fraction)) out << r->block_low_frequency_fraction;
+    out << ',' << (r->block_count > 0 ? r->block_size : 0) << ',' << r->block_count << '\n';
     out.flush();
 }
 

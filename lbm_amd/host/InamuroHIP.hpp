@@ -30,6 +30,7 @@ public:
     InamuroHIP& operator=(const InamuroHIP&) = delete;
     InamuroHIP(InamuroHIP&& o) noexcept
         : ctx(o.ctx), n(o.n), interval_(o.interval_), tol_(o.tol_), limit_(o.limit_), dual_(o.dual_),
+          accel_(o.accel_), fused_(o.fused_), onepass_(o.onepass_), scalar_(o.scalar_),
           pdiag_(std::move(o.pdiag_))
     {
         o.ctx = nullptr;
@@ -44,6 +45,24 @@ public:
     // Segment replay as a captured HIP graph (GPU/InamuroCUDA.hpp:80); on by
     // default, results identical either way.
     void setUsePoissonGraph(bool enabled);
+    // The optional Poisson accelerators (GPU/InamuroCUDA.hpp:60-76, 88),
+    // FP64 only; each call validates as the reference's setter does and
+    // installs the whole set through lbm_set_poisson_accel.  The three variant
+    // flags follow doPressurePoisson's precedence: scalar, onepass, fused.
+    void setUseFusedPoisson(bool enabled);
+    void setUseOnePassPoisson(bool enabled);
+    void setUseScalarPoisson(bool enabled);
+    void setScalarPoissonSourceScale(double scale);
+    void setUseSourceAwareHHInit(bool enabled);
+    void setSourceAwareHHScale(double scale);
+    void setPressureRelaxScale(double scale);
+    void setPoissonFixedPointRelax(double omega);
+    void setUsePoissonAndersonM1(bool enabled);
+    void setPoissonAndersonBetaMax(double value);
+    void setUsePoissonTwoGridCorrection(bool enabled);
+    void setPoissonTwoGridStrength(double value);
+    void setUseFusedBoundaryPressure(bool enabled);
+    void setUsePoissonSpatialDiagnostics(bool enabled);
     // setPoissonDiagnosticsPath (GPU/InamuroCUDA.cu:2610-2627): truncates the
     // file, writes the 20-column header, then appends one row per
     // convergence check (writePoissonDiagnostic, :2634-2682).  An empty path
@@ -65,12 +84,15 @@ public:
 
 private:
     void apply();
+    void applyAccel(const lbm_poisson_accel_t& next);
     void* ctx = nullptr;
     std::size_t n = 0;
     int interval_ = 100;
     double tol_ = 1.0e-3;
     std::uint64_t limit_ = 0;
     int dual_ = 0;
+    lbm_poisson_accel_t accel_{};
+    bool fused_ = false, onepass_ = false, scalar_ = false;
     std::unique_ptr<std::ofstream> pdiag_;  // Poisson diagnostics CSV (heap: stable for the callback)
 };
 

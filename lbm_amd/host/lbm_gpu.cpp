@@ -1,8 +1,9 @@
 // lbm_gpu.cpp -- command-line runner with the reference runner's flags and
 // strict Poisson-contract validation (GPU/main.cpp:18-381).  The time step
-// runs on the MI355X through lbm::Inamuro; flags of the reference's Poisson
-// accelerator variants are parsed and rejected unless left at their default,
-// since this build implements only the strict split contract.  Graph replay of
+// runs on the MI355X through lbm::Inamuro, with the reference's optional
+// Poisson accelerators (--poisson fused|onepass|scalar, --pressure-boundary
+// fused, relaxation scales, Anderson, two-grid, source-aware start, spatial
+// diagnostics) applied as GPU/main.cpp:330-353 applies them.  Graph replay of
 // Poisson segments (--poisson-graph) is always on here.
 #include <algorithm>
 #include <chrono>
@@ -33,12 +34,16 @@ struct RunConfig {
     int poisson_check_interval = 100;
     double poisson_tolerance = 0.001;
     std::uint64_t poisson_iteration_limit = 0;
+    double scalar_source_scale = 2.0;
+    bool source_aware_hh_init = false;
+    double source_aware_hh_scale = 1.0;
     double pressure_relax_scale = 1.0;
     double poisson_fixed_point_relax = 1.0;
+    bool poisson_anderson_m1 = false;
     double poisson_anderson_beta_max = 1.0;
+    bool poisson_two_grid_correction = false;
     double poisson_two_grid_strength = 0.5;
-    bool accelerator = false;
-    std::string accelerator_flag;
+    bool poisson_spatial_diagnostics = false;
     int output_frequency = 100;
     std::set<int> output_steps;
     bool write_output = false;
@@ -53,12 +58,18 @@ struct RunConfig {
 void print_usage(const char* argv0)
 {
     std::cout << "Usage: " << argv0 << " [--mode gpu] [--params FILE] [--steps N] [--precision fp64|mixed]\n"
-              << "       [--poisson split] [--pressure-boundary split]\n"
+              << "       [--poisson split|fused|onepass|scalar] [--pressure-boundary split|fused]\n"
               << "       [--poisson-check-interval 100] [--poisson-tolerance 1e-3]\n"
               << "       [--poisson-iteration-limit 0] (strict unbounded production)\n"
+              << "       [--scalar-source-scale X]\n"
+              << "       [--source-aware-hh-init] [--source-aware-hh-scale X]\n"
+              << "       [--pressure-relax-scale X] [--poisson-fixed-point-relax X]\n"
+              << "       [--poisson-anderson-m1] [--poisson-anderson-beta-max X]\n"
+              << "       [--poisson-two-grid-correction] [--poisson-two-grid-strength X]\n"
               << "       [--output-frequency N] [--output-steps A,B,C] [--write-output] [--no-roofline]\n"
+              << "       [--poisson-graph] [--poisson-detail] [--poisson-diagnostics CSV]\n"
               << "       [--poisson-deterministic-reductions] [--poisson-fixed-point-shadow]\n"
-              << "       [--poisson-dual-residual] [--poisson-diagnostics CSV] [--dump-fields FILE]\n";
+              << "       [--poisson-dual-residual] [--poisson-spatial-diagnostics] [--dump-fields FILE]\n";
 }
 
 std::set<int> parse_output_steps(const std::string& text)
@@ -85,12 +96,6 @@ RunConfig parse_args(int argc, char** argv)
             if (i + 1 >= argc) throw std::runtime_error("missing value for " + arg);
             return argv[++i];
         };
-        auto accel = [&](bool non_default) {
-            if (non_default && !cfg.accelerator) {
-                cfg.accelerator = true;
-                cfg.accelerator_flag = arg;
-            }
-        };
         if (arg == "--help" || arg == "-h") {
             print_usage(argv[0]);
             std::exit(0);
@@ -103,14 +108,14 @@ RunConfig parse_args(int argc, char** argv)
         else if (arg == "--poisson-check-interval") cfg.poisson_check_interval = std::stoi(value());
         else if (arg == "--poisson-tolerance") cfg.poisson_tolerance = std::stod(value());
         else if (arg == "--poisson-iteration-limit") cfg.poisson_iteration_limit = std::stoull(value());
-        else if (arg == "--scalar-source-scale") accel(std::stod(value()) != 2.0);
-        else if (arg == "--source-aware-hh-init") accel(true);
-        else if (arg == "--source-aware-hh-scale") accel(std::stod(value()) != 1.0);
+        else if (arg == "--scalar-source-scale") cfg.scalar_source_scale = std::stod(value());
+        else if (arg == "--source-aware-hh-init") cfg.source_aware_hh_init = true;
+        else if (arg == "--source-aware-hh-scale") cfg.source_aware_hh_scale = std::stod(value());
         else if (arg == "--pressure-relax-scale") cfg.pressure_relax_scale = std::stod(value());
         else if (arg == "--poisson-fixed-point-relax") cfg.poisson_fixed_point_relax = std::stod(value());
-        else if (arg == "--poisson-anderson-m1") accel(true);
+        else if (arg == "--poisson-anderson-m1") cfg.poisson_anderson_m1 = true;
         else if (arg == "--poisson-anderson-beta-max") cfg.poisson_anderson_beta_max = std::stod(value());
-        else if (arg == "--poisson-two-grid-correction") accel(true);
+        else if (arg == "--poisson-two-grid-correction") cfg.poisson_two_grid_correction = true;
         else if (arg == "--poisson-two-grid-strength") cfg.poisson_two_grid_strength = std::stod(value());
         else if (arg == "--output-frequency") { cfg.output_frequency = std::stoi(value()); freq_set = true; }
         else if (arg == "--output-steps") cfg.output_steps = parse_output_steps(value());
@@ -123,7 +128,7 @@ RunConfig parse_args(int argc, char** argv)
             cfg.poisson_dual_residual = true;
             cfg.poisson_fixed_point_shadow = true;
         } else if (arg == "--poisson-diagnostics") cfg.poisson_diagnostics = value();
-        else if (arg == "--poisson-spatial-diagnostics") accel(true);
+        else if (arg == "--poisson-spatial-diagnostics") cfg.poisson_spatial_diagnostics = true;
         else if (arg == "--no-roofline") cfg.print_roofline = false;
         else if (arg == "--dump-fields") cfg.dump_fields = value();
         else throw std::runtime_error("unknown argument: " + arg);
@@ -148,17 +153,20 @@ RunConfig parse_args(int argc, char** argv)
         throw std::runtime_error("--poisson-anderson-beta-max must be non-negative");
     if (!(cfg.poisson_two_grid_strength >= 0.0))
         throw std::runtime_error("--poisson-two-grid-strength must be non-negative");
-    if (cfg.poisson != "split" || cfg.pressure_boundary != "split" || cfg.pressure_relax_scale != 1.0 ||
-        cfg.poisson_fixed_point_relax != 1.0 || cfg.accelerator)
-        throw std::runtime_error(
-            "only the strict split+split Poisson contract with unit relaxation is implemented"
-            + (cfg.accelerator ? " (" + cfg.accelerator_flag + " is a Poisson accelerator variant)" : std::string()));
-    // GPU/main.cpp:229-238 (graph replay itself is always on here; the flag
+    // GPU/main.cpp:229-247 (graph replay itself is always on here; the flag
     // combination is rejected as the reference rejects it).
-    if (cfg.poisson_fixed_point_shadow && cfg.poisson_graph)
+    if (cfg.poisson_fixed_point_shadow &&
+        (cfg.poisson != "split" || cfg.pressure_boundary != "split" || cfg.source_aware_hh_init ||
+         cfg.poisson_graph || cfg.poisson_anderson_m1 || cfg.poisson_two_grid_correction ||
+         cfg.pressure_relax_scale != 1.0 || cfg.poisson_fixed_point_relax != 1.0))
         throw std::runtime_error(
             "fixed-point residual requires GPU split+split, unit relaxation, "
             "and source-aware/graph/Anderson/two-grid disabled");
+    if (cfg.poisson_dual_residual && cfg.poisson_spatial_diagnostics)
+        throw std::runtime_error(
+            "dual-residual exit requires check_interval=100, tolerance=1e-3, "
+            "poisson_iteration_limit=0, deterministic reductions, and no "
+            "spatial diagnostics");
     if (!cfg.precision.empty() && cfg.precision != "fp64" && cfg.precision != "mixed")
         throw std::runtime_error("--precision must be fp64 or mixed");
     // Steps and output frequency from params.in lines 3-4, GPU/main.cpp:98-118.
@@ -190,10 +198,17 @@ int main(int argc, char** argv)
         P.gpu_tree_init = true;  // the GPU runner uses the GPU-tree constructor (GPU/Inamuro.cpp:491-566)
         if (!cfg.precision.empty()) P.precision = cfg.precision;
         std::cout << "LBM runner mode=gpu device=MI355X precision=" << P.precision
-                  << " poisson=split pressure_boundary=split poisson_check_interval=100"
+                  << " poisson=" << cfg.poisson << " pressure_boundary=" << cfg.pressure_boundary
+                  << " poisson_check_interval=100"
                   << " poisson_tolerance=" << cfg.poisson_tolerance << " poisson_iteration_limit=0 (unbounded)"
                   << " poisson_fixed_point_shadow=" << (cfg.poisson_fixed_point_shadow ? "yes" : "no")
                   << " poisson_dual_residual=" << (cfg.poisson_dual_residual ? "yes" : "no")
+                  << " poisson_fixed_point_relax=" << cfg.poisson_fixed_point_relax
+                  << " poisson_anderson_m1=" << (cfg.poisson_anderson_m1 ? "yes" : "no")
+                  << " poisson_anderson_beta_max=" << cfg.poisson_anderson_beta_max
+                  << " poisson_two_grid_correction=" << (cfg.poisson_two_grid_correction ? "yes" : "no")
+                  << " poisson_two_grid_strength=" << cfg.poisson_two_grid_strength
+                  << " poisson_spatial_diagnostics=" << (cfg.poisson_spatial_diagnostics ? "yes" : "no")
                   << " params=" << cfg.params << " steps=" << cfg.steps << std::endl;
         lbm::Inamuro solver(P);
         lbm::InamuroHIP& gpu = solver.gpu();
@@ -202,6 +217,21 @@ int main(int argc, char** argv)
         gpu.setPoissonIterationLimit(cfg.poisson_iteration_limit);
         gpu.setEnablePoissonFixedPointShadow(cfg.poisson_fixed_point_shadow);
         gpu.setUsePoissonDualResidual(cfg.poisson_dual_residual);
+        // GPU/main.cpp:330-351.
+        gpu.setUseFusedPoisson(cfg.poisson == "fused");
+        gpu.setUseOnePassPoisson(cfg.poisson == "onepass");
+        gpu.setUseScalarPoisson(cfg.poisson == "scalar");
+        gpu.setScalarPoissonSourceScale(cfg.scalar_source_scale);
+        gpu.setUseSourceAwareHHInit(cfg.source_aware_hh_init);
+        gpu.setSourceAwareHHScale(cfg.source_aware_hh_scale);
+        gpu.setPressureRelaxScale(cfg.pressure_relax_scale);
+        gpu.setPoissonFixedPointRelax(cfg.poisson_fixed_point_relax);
+        gpu.setUsePoissonAndersonM1(cfg.poisson_anderson_m1);
+        gpu.setPoissonAndersonBetaMax(cfg.poisson_anderson_beta_max);
+        gpu.setUsePoissonTwoGridCorrection(cfg.poisson_two_grid_correction);
+        gpu.setPoissonTwoGridStrength(cfg.poisson_two_grid_strength);
+        gpu.setUseFusedBoundaryPressure(cfg.pressure_boundary == "fused");
+        gpu.setUsePoissonSpatialDiagnostics(cfg.poisson_spatial_diagnostics);
         lbm::Inamuro::OutputConfig oc;
         oc.enable_tecplot = true;
         oc.output_dir = "out/";
@@ -232,7 +262,16 @@ int main(int argc, char** argv)
                       << "; Poisson iterations/step " << static_cast<double>(pf.poisson_iterations) / pf.steps
                       << ", ms/iteration " << iter_ms << std::endl;
             if (cfg.print_roofline) {
-                const double bpc = P.precision == "mixed" ? 144.0 : 272.0;
+                // Algorithmic bytes per cell of one iteration: h read and
+                // written, p read and written, tau_h and the source read;
+                // scalar Jacobi moves p, rho and the source only; an
+                // Anderson(1) iteration also reads the previous residual in
+                // the map and reads three / writes two distributions to mix.
+                double bpc = P.precision == "mixed" ? 144.0 : 272.0;
+                if (cfg.poisson == "scalar")
+                    bpc = 32.0;
+                else if (cfg.poisson == "onepass" && cfg.poisson_anderson_m1)
+                    bpc = 992.0;
                 std::cout << "Poisson iteration: " << bpc << " algorithmic bytes/cell, "
                           << bpc * cells / (iter_ms * 1e-3) / 1e9 << " GB/s = "
                           << 100.0 * bpc * cells / (iter_ms * 1e-3) / 8.0e12 << "% of 8 TB/s" << std::endl;

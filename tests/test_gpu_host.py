@@ -75,6 +75,46 @@ def test_runner_matches_oracle_with_gpu_tree_initialisation(tmp_path):
         assert rel_linf(got[i], o.get(k)) <= 1e-12, k
 
 
+def test_runner_applies_the_poisson_accelerators(tmp_path):
+    """lbm_gpu --poisson onepass --poisson-anderson-m1
+    --poisson-two-grid-correction --poisson-spatial-diagnostics, as
+    GPU/main.cpp:330-351 applies them, against the oracle's restatement: the
+    same iteration counts, fields within 1e-10, and the CSV's block columns
+    filled (4, 256 blocks, the spatial fraction)."""
+    P = po.Params(lx=16, ly=32, lz=32, DD=8.0, max_time_steps=2)
+    pfile = tmp_path / "params.in"
+    pfile.write_text(P.params_in())
+    dump, diag = tmp_path / "fields.bin", tmp_path / "poisson.csv"
+    r = subprocess.run([LBM_GPU, "--params", str(pfile), "--poisson", "onepass", "--poisson-anderson-m1",
+                        "--poisson-two-grid-correction", "--poisson-two-grid-strength", "0.3",
+                        "--poisson-spatial-diagnostics", "--pressure-boundary", "fused",
+                        "--dump-fields", str(dump), "--poisson-diagnostics", str(diag)],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert "poisson=onepass pressure_boundary=fused" in r.stdout
+    assert "poisson_anderson_m1=yes" in r.stdout and "poisson_two_grid_correction=yes" in r.stdout
+    o = po.Oracle(P, threads=4)
+    o.place_droplet(0.0, P.ly / 2.0, P.lz / 2.0 + P.DD, vz=-0.035, profile="sharp")
+    o.place_droplet(0.0, P.ly / 2.0, P.lz / 2.0 - P.DD, vz=0.035, profile="sharp")
+    o.set_poisson_accel(variant="onepass", anderson_m1=1, two_grid=1, two_grid_strength=0.3,
+                        spatial_diagnostics=1, fused_boundary=1)
+    iters = o.step(2)
+    d = o.diag()
+    with open(diag) as fh:
+        header = fh.readline().strip().split(",")
+        rows = [dict(zip(header, ln.split(","))) for ln in fh.read().splitlines()]
+    want = [(s_ + 1, k) for s_, n in enumerate(iters) for k in range(100, n + 1, 100)]
+    assert [(int(r_["step"]), int(r_["iteration"])) for r_ in rows] == want
+    for r_ in rows:
+        assert (r_["block_size"], r_["block_count"]) == ("4", "256")
+        assert 0.0 <= float(r_["block_low_frequency_fraction"]) <= 1.0
+    assert float(rows[-1]["block_low_frequency_fraction"]) == pytest.approx(d.block_low_frequency_fraction,
+                                                                            rel=1e-9)
+    got = np.fromfile(dump, dtype=np.float64).reshape(6, P.lz, P.ly, P.lx)
+    for i, k in enumerate(FIELDS):
+        assert rel_linf(got[i], o.get(k)) <= 1e-10, k
+
+
 def test_solver_writes_reference_step_diagnostics(tmp_path):
     P = po.Params(lx=16, ly=32, lz=32, DD=8.0, max_time_steps=3, output_frequency=2)
     out = tmp_path / "out"

@@ -24,10 +24,19 @@ def run(args, cwd=None):
     (["--pressure-boundary", "x"], "--pressure-boundary must be split or fused"),
     (["--poisson-fixed-point-relax", "0"], "--poisson-fixed-point-relax must be positive"),
     (["--poisson-anderson-beta-max", "-1"], "--poisson-anderson-beta-max must be non-negative"),
-    (["--poisson", "onepass"], "only the strict split+split Poisson contract"),
-    (["--poisson-anderson-m1"], "--poisson-anderson-m1 is a Poisson accelerator variant"),
+    (["--poisson-two-grid-strength", "-0.5"], "--poisson-two-grid-strength must be non-negative"),
     (["--poisson-graph", "--poisson-fixed-point-shadow"],
      "fixed-point residual requires GPU split+split, unit relaxation, and source-aware/graph/Anderson/two-grid disabled"),
+    (["--poisson", "onepass", "--poisson-dual-residual"],
+     "fixed-point residual requires GPU split+split, unit relaxation, and source-aware/graph/Anderson/two-grid disabled"),
+    (["--pressure-boundary", "fused", "--poisson-fixed-point-shadow"],
+     "fixed-point residual requires GPU split+split"),
+    (["--poisson-two-grid-correction", "--poisson-fixed-point-shadow"],
+     "fixed-point residual requires GPU split+split"),
+    (["--pressure-relax-scale", "0.9", "--poisson-dual-residual"], "fixed-point residual requires GPU split+split"),
+    (["--poisson-spatial-diagnostics", "--poisson-dual-residual"],
+     "dual-residual exit requires check_interval=100, tolerance=1e-3, poisson_iteration_limit=0, "
+     "deterministic reductions, and no spatial diagnostics"),
     (["--bogus"], "unknown argument: --bogus"),
     (["--steps"], "missing value for --steps"),
     (["--output-steps", "3,-1"], "--output-steps values must be non-negative"),
@@ -65,5 +74,8 @@ def test_solver_rejects_unknown_precision_key(tmp_path):
 def test_help_lists_the_contract_flags():
     r = run(["--help"])
     assert r.returncode == 0
-    for flag in ("--poisson-check-interval 100", "--poisson-tolerance 1e-3", "--poisson-dual-residual"):
+    for flag in ("--poisson-check-interval 100", "--poisson-tolerance 1e-3", "--poisson-dual-residual",
+                 "--poisson split|fused|onepass|scalar", "--pressure-boundary split|fused",
+                 "--poisson-anderson-m1", "--poisson-two-grid-correction", "--source-aware-hh-init",
+                 "--poisson-spatial-diagnostics", "--scalar-source-scale"):
         assert flag in r.stdout
