@@ -430,8 +430,11 @@ void launch_push(Ctx& c, int mode, int cfg)
 {
     const Grid& g = c.g;
     if (!c.mixed) {
+        // Only the split-family accelerators reach this kernel; its
+        // relaxation scale is 1 on the strict path.
         lbm::f64::poisson_iter(c.stream, mode, g, D<double>(c.h[c.hcur]), D<double>(c.h[1 - c.hcur]),
-                               c.p[c.pc], D<double>(c.tau), D<double>(c.src), c.dst, cfg);
+                               c.p[c.pc], D<double>(c.tau), D<double>(c.src), c.dst, cfg,
+                               c.accel.pressure_relax_scale);
     } else {
         // Decodes with base pb.  The first iteration of a segment (equilibrium,
         // explicit or gauged entry) writes new bases into the other buffer;
@@ -569,17 +572,27 @@ void poisson_check(Ctx& c)
 
 // ---- the reference's optional Poisson accelerators -------------------------
 
-// Settings whose effect differs from the strict split map: these run the
-// accelerated loop.  The fused variant, the fused boundary kernel, Anderson
-// without onepass and the fixed-point relaxation without onepass compute the
-// split values bit for bit (GPU/InamuroCUDA.cu:2216-2323), so they keep the
-// strict kernel.
-bool accel_on(const Ctx& c)
+// Which loop an accelerator setting runs on.  Onepass (with or without
+// Anderson) and scalar Jacobi need their own state (explicit h, p written by
+// every iteration): the pull loop.  The split and fused variants compute the
+// split map, so the strict push kernel carries them, with the relaxation
+// scale as a kernel variant, the source-aware start as an explicit first h,
+// and the two-grid correction and spatial diagnostics in its check.  The
+// fused boundary kernel, and Anderson or the fixed-point relaxation without
+// onepass, change no value (GPU/InamuroCUDA.cu:2216-2323).
+bool accel_pull(const Ctx& c)
+{
+    return c.accel.variant == LBM_POISSON_ONEPASS || c.accel.variant == LBM_POISSON_SCALAR;
+}
+
+bool accel_push(const Ctx& c)
 {
     const lbm_poisson_accel_t& a = c.accel;
-    return a.variant == LBM_POISSON_ONEPASS || a.variant == LBM_POISSON_SCALAR ||
-           a.pressure_relax_scale != 1.0 || a.source_aware_hh_init || a.two_grid || a.spatial_diagnostics;
+    return !accel_pull(c) &&
+           (a.pressure_relax_scale != 1.0 || a.source_aware_hh_init || a.two_grid || a.spatial_diagnostics);
 }
+
+bool accel_on(const Ctx& c) { return accel_pull(c) || accel_push(c); }
 
 // Settings the reference's fixed-point residual refuses
 // (GPU/InamuroCUDA.cu:1846-1857), effective or not.
@@ -591,6 +604,16 @@ bool accel_requested(const Ctx& c)
 }
 
 double* acc_h(Ctx& c, int role) { return role == 2 ? c.acc_pi : D<double>(c.h[role]); }
+
+// The 4^3 block sums of the two-grid correction and spatial diagnostics.
+double* acc_blocks(Ctx& c)
+{
+    if (!c.acc_blocks) {
+        int nbx = 0, nby = 0;
+        c.acc_blocks = dalloc(static_cast<size_t>(lbm::acc::block_count(c.g, &nbx, &nby)));
+    }
+    return c.acc_blocks;
+}
 double* acc_pbuf(Ctx& c, int which) { return which == 0 ? c.p[c.pc] : c.acc_p; }
 bool acc_anderson(const Ctx& c) { return c.accel.anderson_m1 && c.accel.variant == LBM_POISSON_ONEPASS; }
 
@@ -603,8 +626,7 @@ void accel_begin(Ctx& c)
     const size_t n = c.n();
     const bool scalar = a.variant == LBM_POISSON_SCALAR;
     if (!c.acc_p) c.acc_p = dalloc(n);
-    int nbx = 0, nby = 0;
-    if (!c.acc_blocks) c.acc_blocks = dalloc(static_cast<size_t>(lbm::acc::block_count(c.g, &nbx, &nby)));
+    acc_blocks(c);
     if (acc_anderson(c) && !c.acc_pi) {
         c.acc_pi = dalloc(lbm::Q * n);
         c.acc_pr = dalloc(lbm::Q * n);
@@ -736,6 +758,8 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
 {
     hipStream_t s = c.stream;
     const bool acc = accel_on(c);
+    const bool pull = accel_pull(c);
+    const bool blocks = acc && (c.accel.two_grid || c.accel.spatial_diagnostics);
     // doPressurePoisson's refusals (GPU/InamuroCUDA.cu:1846-1865), made
     // before the step starts so that a refused call changes nothing.
     if (c.dual != 0 && accel_requested(c))
@@ -764,10 +788,19 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
 
     int hmode = c.hs == HState::Equilibrium ? lbm::H_EQUILIBRIUM : lbm::H_EXPLICIT;
     const uint64_t forced = c.sched_pos < c.schedule.size() ? c.schedule[c.sched_pos] : 0;
-    if (acc)
+    if (pull) {
         accel_begin(c);
-    else
+    } else {
         choose_push_config(c);
+        if (acc && c.accel.source_aware_hh_init) {
+            // sourceAwareHHInitKernel (GPU/InamuroCUDA.cu:2162-2172): an
+            // explicit post-boundary h, read as uploaded h is.
+            lbm::acc::source_init(s, c.g, c.accel.source_aware_hh_scale, c.p[c.pc], D<double>(c.tau),
+                                  D<double>(c.src), D<double>(c.h[c.hcur]));
+            HIPCHK(hipGetLastError());
+            hmode = lbm::H_EXPLICIT;
+        }
+    }
     uint64_t iter = 0;
     bool first_segment = true;
     for (;;) {
@@ -775,16 +808,22 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
         if (iter > std::numeric_limits<uint64_t>::max() - static_cast<uint64_t>(c.check_interval))
             throw ApiError{LBM_ERR_OVERFLOW, "Poisson iteration counter overflow"};
         iter += static_cast<uint64_t>(c.check_interval);
-        if (acc)
+        if (pull)
             accel_segment(c);
         else
             poisson_segment(c, hmode);
         HIPCHK(hipGetLastError());
         record(c, 5);
-        if (acc)
+        if (pull) {
             accel_check(c);
-        else
+        } else if (blocks) {
+            double* hs = D<double>(c.h[c.hcur]);
+            lbm::acc::check_streamed(s, c.g, c.ph, c.tol, c.accel.spatial_diagnostics, c.accel.two_grid,
+                                     c.accel.two_grid_strength, hs, hs, c.p[c.pc], c.p_prev, acc_blocks(c),
+                                     c.partials, c.dst);
+        } else {
             poisson_check(c);
+        }
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(c.hst, c.dst, sizeof(DevState), hipMemcpyDeviceToHost, s));
         record(c, 6);
@@ -848,7 +887,7 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
             rec.converged = d.converged;
             rec.block_low_frequency_fraction = std::numeric_limits<double>::quiet_NaN();
             rec.anderson_alpha = std::numeric_limits<double>::quiet_NaN();
-            if (acc && (c.accel.two_grid || c.accel.spatial_diagnostics)) {
+            if (blocks) {
                 int nbx = 0, nby = 0;
                 rec.block_count = lbm::acc::block_count(c.g, &nbx, &nby);
                 rec.block_size = 4;
@@ -856,7 +895,7 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
                 if (c.accel.spatial_diagnostics)
                     rec.block_low_frequency_fraction = st.e1 > 0.0 ? st.acc_block_abs / st.e1 : 0.0;
             }
-            if (acc && acc_anderson(c) && c.acc_iters >= 2) rec.anderson_alpha = st.acc_alpha;
+            if (pull && acc_anderson(c) && c.acc_iters >= 2) rec.anderson_alpha = st.acc_alpha;
             c.check_observer(c.check_user, &rec);
         }
         if (hook) hook->fn(c, iter, hook->arg);
@@ -873,7 +912,7 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
         }
         hmode = lbm::H_GAUGED;
     }
-    if (acc) accel_end(c);
+    if (pull) accel_end(c);
     c.last_iterations = iter;
     c.perf.poisson_iterations += iter;
     c.history.push_back(iter);

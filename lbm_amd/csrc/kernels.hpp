@@ -102,9 +102,14 @@ void projected_finalize(hipStream_t s, const double* parts, int nparts, DevState
 // One Poisson iteration; config selects the launch shape (block size and
 // load hint) among PUSH_CONFIGS, with results identical for all of them.
 constexpr int PUSH_CONFIGS = 3;
+// relax != 1 selects the reference's pressure relaxation scale (an
+// accelerator; the strict map is relax = 1, a kernel of its own).
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, double* hout,
                   const double* p, const double* tau, const double* src, const DevState* st,
-                  int config);
+                  int config, double relax = 1.0);
+// getp of a check (p = sum_q of the streamed h under the slip view) with the
+// block partial sums of p for the gauge, red_blocks(g.n) of them.
+void getp_partials(hipStream_t s, const Grid& g, const double* hin, double* p, double* parts);
 void check(hipStream_t s, const Grid& g, const Phys& ph, double* p, double* p_prev,
            const double* hin, const double* tau, const double* src, int dual, double tol,
            double* partials, DevState* st);
@@ -215,6 +220,15 @@ int block_count(const Grid& g, int* nbx, int* nby);
 void check(hipStream_t s, const Grid& g, const Phys& ph, double tol, int blocks_needed,
            int spatial, int two_grid, double strength, double* p, double* p_prev, double* h,
            double* blocks, double* partials, DevState* st);
+// The same check on the strict kernel's state (streamed h, gauge applied
+// lazily by the next iteration): p = getp of the streamed h, the gauge and
+// relative change with block sums, the spatial sum and the two-grid
+// correction, which adds E correction to every stored population of the
+// cell (slip partners share their weight, so the slip view moves by the
+// same amount).
+void check_streamed(hipStream_t s, const Grid& g, const Phys& ph, double tol, int spatial, int two_grid,
+                    double strength, const double* hin_const, double* h_stored, double* p, double* p_prev,
+                    double* blocks, double* partials, DevState* st);
 
 } // namespace acc
 

@@ -378,5 +378,26 @@ void check(hipStream_t s, const Grid& g, const Phys& ph, double tol, int blocks_
         k_acc_two_grid<<<grid_blocks(g.n), BLOCK, 0, s>>>(g, nbx, nby, strength, blocks, st, p, p_prev, h);
 }
 
+void check_streamed(hipStream_t s, const Grid& g, const Phys& ph, double tol, int spatial, int two_grid,
+                    double strength, const double* hin_const, double* h_stored, double* p, double* p_prev,
+                    double* blocks, double* partials, DevState* st)
+{
+    const int nb = red_blocks(g.n);
+    double* parts_sum = partials;
+    double* parts_err = partials + RED_BLOCKS;
+    f64::getp_partials(s, g, hin_const, p, parts_sum);
+    f64::gauge_finalize(s, ph, parts_sum, nb, st);
+    int nbx = 0, nby = 0;
+    const int nblk = block_count(g, &nbx, &nby);
+    const int wg = (nblk + 3) / 4;
+    const int nparts = wg < RED_BLOCKS ? wg : RED_BLOCKS;
+    k_acc_gauge_error_blocks<<<nparts, BLOCK, 0, s>>>(g, nbx, nby, nblk, p, p_prev, st, blocks, parts_err);
+    f64::check_finalize(s, parts_err, nullptr, nparts, 0, tol, st);
+    if (spatial) k_acc_block_abs<<<1, BLOCK, 0, s>>>(blocks, nblk, st);
+    if (two_grid)
+        k_acc_two_grid<<<grid_blocks(g.n), BLOCK, 0, s>>>(g, nbx, nby, strength, blocks, st, p, p_prev,
+                                                           h_stored);
+}
+
 } // namespace acc
 } // namespace lbm

@@ -393,11 +393,14 @@ __device__ __forceinline__ void load_streamed(const double* __restrict__ hin, in
 // NT: the streamed h is loaded with a non-temporal hint (it is dead after
 // this read).  PB: threads per block.  Which (PB, NT) pair is fastest depends
 // on the grid's shape; the caller picks one of PUSH_CONFIGS.
-template <int MODE, int NT, int PB>
+// RELAX: the reference GPU solver's pressure relaxation scale s
+// (collisionPressureKernel, GPU/InamuroCUDA.cu:503-537), h - s (h - E p) /
+// tau_h - (E/3) S; the strict map is RELAX = 0.
+template <int MODE, int NT, int PB, int RELAX>
 __global__ __launch_bounds__(PB) void k_poisson_push(
     Grid g, const double* __restrict__ hin, double* __restrict__ hout,
     const double* __restrict__ pbuf, const double* __restrict__ tau,
-    const double* __restrict__ src, const DevState* __restrict__ st)
+    const double* __restrict__ src, const DevState* __restrict__ st, double relax)
 {
     const int64_t c = static_cast<int64_t>(blockIdx.x) * PB + threadIdx.x;
     if (c >= g.n) return;
@@ -434,7 +437,8 @@ __global__ __launch_bounds__(PB) void k_poisson_push(
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const double hequ = EW[q] * p;
-        const double o = hv[q] - (hv[q] - hequ) / tauh - (EW[q] / 3.0) * sc;
+        const double o = RELAX ? hv[q] - relax * (hv[q] - hequ) / tauh - (EW[q] / 3.0) * sc
+                               : hv[q] - (hv[q] - hequ) / tauh - (EW[q] / 3.0) * sc;
         const int dx = CX[q] == 1 ? xp : (CX[q] == -1 ? xm : x);
         const int dy = CY[q] == 1 ? yp : (CY[q] == -1 ? ym : y);
         const int dz = CZ[q] == 1 ? zp : (CZ[q] == -1 ? zm : z);
@@ -786,21 +790,28 @@ void projected_finalize(hipStream_t s, const double* parts, int nparts, DevState
 
 void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, double* hout,
                   const double* p, const double* tau, const double* src, const DevState* st,
-                  int config)
+                  int config, double relax)
 {
     // PUSH_CONFIGS: 0 = 256 threads with non-temporal loads, 1 = 256 plain,
     // 2 = 768 plain.
-#define LBM_PUSH_AT(M, NT, PB)                                                                     \
-    k_poisson_push<M, NT, PB><<<static_cast<int>((g.n + PB - 1) / PB), PB, 0, s>>>(g, hin, hout, p, \
-                                                                                  tau, src, st)
-#define LBM_PUSH(M)                                                                          \
+#define LBM_PUSH_AT(M, NT, PB, R)                                                                     \
+    k_poisson_push<M, NT, PB, R><<<static_cast<int>((g.n + PB - 1) / PB), PB, 0, s>>>(g, hin, hout, p, \
+                                                                                     tau, src, st, relax)
+#define LBM_PUSH_R(M, R)                                                                     \
     do {                                                                                     \
         if (config == 1)                                                                     \
-            LBM_PUSH_AT(M, 0, 256);                                                          \
+            LBM_PUSH_AT(M, 0, 256, R);                                                       \
         else if (config == 2)                                                                \
-            LBM_PUSH_AT(M, 0, 768);                                                          \
+            LBM_PUSH_AT(M, 0, 768, R);                                                       \
         else                                                                                 \
-            LBM_PUSH_AT(M, 1, 256);                                                          \
+            LBM_PUSH_AT(M, 1, 256, R);                                                       \
+    } while (0)
+#define LBM_PUSH(M)                  \
+    do {                             \
+        if (relax != 1.0)            \
+            LBM_PUSH_R(M, 1);        \
+        else                         \
+            LBM_PUSH_R(M, 0);        \
     } while (0)
     switch (mode) {
     case H_EQUILIBRIUM: LBM_PUSH(H_EQUILIBRIUM); break;
@@ -809,7 +820,13 @@ void poisson_iter(hipStream_t s, int mode, const Grid& g, const double* hin, dou
     default: LBM_PUSH(H_STREAMED); break;
     }
 #undef LBM_PUSH
+#undef LBM_PUSH_R
 #undef LBM_PUSH_AT
+}
+
+void getp_partials(hipStream_t s, const Grid& g, const double* hin, double* p, double* parts)
+{
+    k_getp_sum<<<red_blocks(g.n), BLOCK, 0, s>>>(g, hin, p, parts);
 }
 
 void check(hipStream_t s, const Grid& g, const Phys& ph, double* p, double* p_prev,
