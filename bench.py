@@ -107,15 +107,21 @@ def make_sim(hip, wl, precision):
 def run_precision(hip, wl, precision, steps, warmup):
     sim, P = make_sim(hip, wl, precision)
     sim.set_timing(True)
-    for _ in range(warmup):
+    t_start = time.perf_counter()
+    for k in range(warmup):
         sim.performTimeStepGPU(1)
+        # progress on stderr as each step completes (the getter waits for it)
+        print(f"{precision} warm-up step {k + 1}/{warmup}: {sim.getLastPoissonDiagnostics().iterations} "
+              f"Poisson iterations, {time.perf_counter() - t_start:.1f} s", file=sys.stderr, flush=True)
     sim.synchronize()
     sim.reset_perf()
     iters = []
     t0 = time.perf_counter()
-    for _ in range(steps):
+    for k in range(steps):
         sim.performTimeStepGPU(1)
         iters.append(int(sim.getLastPoissonDiagnostics().iterations))
+        print(f"{precision} step {k + 1}/{steps}: {iters[-1]} Poisson iterations, "
+              f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
     sim.synchronize()
     t1 = time.perf_counter()
     pf = sim.perf()
@@ -458,10 +464,12 @@ def cpu_baseline(hip, wl, workload, n_it_per_step):
         for k, v in fields.items():
             o.set(k, v)
     cells = lx * ly * lz
+    print(f"cpu baseline: oracle on one core, {lx}x{ly}x{lz} state ready", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
     for st in ("collision", "stream_fg", "bc_fg", "macro"):
         o.stage(st)
     t1 = time.perf_counter()
+    print(f"cpu baseline: step without the solve {t1 - t0:.1f} s", file=sys.stderr, flush=True)
     k = 2
     o.poisson_iterations(k)
     t2 = time.perf_counter()

@@ -154,7 +154,8 @@ typedef struct lbm_adjoint_stats_s {
     int64_t store_slots;             /* Poisson iterates recorded per block */
     uint64_t forward_iterations;     /* Poisson iterations of the forward pass */
     uint64_t recomputed_iterations;  /* Poisson iterations re-run in the reverse sweep */
-    uint64_t adjoint_iterations;     /* adjoint Poisson iterations */
+    uint64_t adjoint_iterations;     /* adjoint Poisson iterations (the last step's
+                                        solve carries no adjoint and is skipped) */
     double forward_ms;               /* forward pass with checkpoint saves */
     double reverse_ms;               /* reverse sweep, replays and recomputation included */
     /* Convergence of the reverse Poisson solves, as the forward check measures
@@ -281,14 +282,15 @@ int lbm_get_poisson_accel(const void* ctx, lbm_poisson_accel_t* out);
 int lbm_set_poisson_graph(void* ctx, int enabled);
 
 /* Launch shape of the Poisson iteration kernel (no counterpart in the
- * reference): -1, the default, times the candidates on the context's own
- * buffers before the first solve and keeps the fastest for this grid; a
- * candidate's index fixes it (FP64, 0..2: 256 threads with non-temporal loads
- * of h, 256 plain, 768 plain; mixed, 0..3: 1024 threads with non-temporal
- * loads, 512 with, 512 plain, 1024 plain).
+ * reference): -1, the default, tunes on the context's own solves -- the
+ * check-interval segments that start from a check run the candidates in
+ * turn, timed with HIP events, three segments each, and the fastest is kept
+ * for this grid; a candidate's index fixes it (FP64, 0..2: 256 threads with
+ * non-temporal loads of h, 256 plain, 768 plain; mixed, 0..3: 1024 threads
+ * with non-temporal loads, 512 with, 512 plain, 1024 plain).
  * Every choice gives bit-identical results; only the speed differs with the
- * grid's shape.  lbm_get_poisson_launch reports the one in use (-1 before the
- * first solve when choosing automatically). */
+ * grid's shape.  lbm_get_poisson_launch reports the one in use (-1 while
+ * still tuning). */
 int lbm_set_poisson_launch(void* ctx, int config);
 int lbm_get_poisson_launch(const void* ctx, int* config);
 
@@ -378,7 +380,10 @@ int lbm_mass_noalloc(const double* d_field, int64_t n, double* d_sum, void* stre
  * iteration counts are `iterations[0..steps-1]` when given (positive multiples
  * of the check interval), otherwise those the forward pass converges to; the
  * gradient is that of the step with these counts frozen, transposing every
- * executed iteration and gauge.  State is checkpointed every
+ * executed iteration and gauge that J depends on (J reads phi_T, which is
+ * formed before the last step's Poisson solve and velocity correction, so
+ * the reverse sweep neither recomputes nor transposes that solve: its
+ * adjoint is identically zero).  State is checkpointed every
  * `checkpoint_interval` steps and recomputed in the reverse sweep.  On return
  * the context holds the state at the end of the forward pass. */
 int lbm_adjoint(void* ctx, const double* phi0, const double* target, int32_t steps,

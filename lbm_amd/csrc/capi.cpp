@@ -104,6 +104,13 @@ struct Ctx {
     size_t sched_pos = 0;
     int64_t adj_store_bytes = 0;     // device bytes for recorded Poisson iterates (0 = automatic)
     int adj_objective = lbm::ADJ_OBJ_MISFIT;
+    // Threads per block of the adjoint Poisson kernel for FP64 / FP32 adjoint
+    // state (-1: timed on the first reverse iterations of the next call).
+    int adj_bs[2] = {-1, -1};
+    // Launch shape of the reverse sweep's recomputation of a block, whose
+    // iterates go to fresh slots instead of a cache-resident ping-pong pair
+    // (-1: timed on its first iterations, like adj_bs).
+    int rec_cfg = -1;
     bool adj_state_f32 = false;      // adjoint Poisson carriers stored in FP32
     lbm_check_fn check_observer = nullptr;  // per-check records (Poisson diagnostics CSV)
     void* check_user = nullptr;
@@ -114,17 +121,20 @@ struct Ctx {
     // Captured Poisson segments (setUsePoissonGraph): one graph per entry
     // mode and buffer parity, each holding check_interval iterations.
     struct SegGraph {
-        int mode, hcur, pb, ci;
+        int mode, hcur, pb, ci, cfg;
         hipGraphExec_t exec;
         int hcur_after, pb_after;
     };
     bool use_graphs = true;
     // Launch shape of the Poisson push kernel (block size and load hint, one
     // of the precision's PUSH_CONFIGS; every one gives the same results).
-    // push_request < 0: time the candidates at the first solve and keep the
-    // fastest (push_cfg < 0 until then).
+    // push_request < 0: tune on the context's own segments and keep the
+    // fastest (push_cfg < 0 until then, next_segment_config).
     int push_request = -1;
     int push_cfg = -1;
+    int seg_cfg = 0;               // shape of the segment being launched
+    std::vector<float> tune_best;  // tuning: each candidate's fastest segment (ms)
+    int tune_done = 0;             // timed tuning segments so far
     std::vector<SegGraph> graphs;
 
     // The reference's optional Poisson accelerators (FP64; kernels_accel.hip).
@@ -155,7 +165,7 @@ struct Ctx {
 
     bool timing = false;
     lbm_perf_t perf{};
-    hipEvent_t ev[10] = {};
+    hipEvent_t ev[12] = {};
 
     std::string err;
 
@@ -445,43 +455,39 @@ void launch_push(Ctx& c, int mode, int cfg)
     }
 }
 
-// Picks the push kernel's launch shape for this grid.  With push_request < 0
-// each candidate runs the steady-state (streamed) iteration from h[hcur] into
-// h[1 - hcur] a few times under HIP events, and the fastest is kept.  This
-// writes nothing the next real iteration does not overwrite in full (the push
-// writes every population of every cell of the output buffer; streamed
-// iterations write no bases), so the state is unchanged.  Called outside graph
-// capture, before the first solve's first segment.
-void choose_push_config(Ctx& c)
+// Launch shape of the next segment.  A requested shape is used as is.
+// Otherwise the context tunes on its own solves: segments that start from a
+// check (the bulk of every long solve) run the candidates in turn, each timed
+// with HIP events around its launch or graph replay, until every candidate
+// has TUNE_ROUNDS timed segments; the candidate with the fastest segment is
+// kept from then on.  The timing sees the solver's real ping-pong buffers,
+// cache state and launch mode.  Every candidate computes the same values, so
+// switching shapes between segments changes no result.
+constexpr int TUNE_ROUNDS = 3;
+
+int next_segment_config(Ctx& c, int hmode, bool* timed)
 {
-    if (c.push_cfg >= 0) return;
+    *timed = false;
+    if (c.push_request >= 0) return c.push_request;
+    if (c.push_cfg >= 0) return c.push_cfg;
     const int ncfg = c.mixed ? lbm::mx::PUSH_CONFIGS : lbm::f64::PUSH_CONFIGS;
-    if (c.push_request >= 0) {
-        c.push_cfg = c.push_request;
-        return;
-    }
-    // Three interleaved rounds, each candidate's best round counts: run-to-run
-    // spread is a few percent, about what separates the candidates.
-    constexpr int ROUNDS = 3, WARM = 2, TIMED = 5;
-    std::vector<float> best(static_cast<size_t>(ncfg), std::numeric_limits<float>::max());
-    for (int round = 0; round < ROUNDS; ++round) {
-        for (int cfg = 0; cfg < ncfg; ++cfg) {
-            for (int k = 0; k < WARM; ++k) launch_push(c, lbm::H_STREAMED, cfg);
-            HIPCHK(hipEventRecord(c.ev[3], c.stream));
-            for (int k = 0; k < TIMED; ++k) launch_push(c, lbm::H_STREAMED, cfg);
-            HIPCHK(hipEventRecord(c.ev[9], c.stream));
-            HIPCHK(hipGetLastError());
-            HIPCHK(hipEventSynchronize(c.ev[9]));
-            best[static_cast<size_t>(cfg)] = std::min(best[static_cast<size_t>(cfg)], elapsed(c.ev[3], c.ev[9]));
-        }
-    }
-    c.push_cfg = static_cast<int>(std::min_element(best.begin(), best.end()) - best.begin());
+    if (c.tune_best.empty()) c.tune_best.assign(static_cast<size_t>(ncfg), std::numeric_limits<float>::max());
+    *timed = hmode == lbm::H_GAUGED;
+    return c.tune_done % ncfg;
+}
+
+void tune_record(Ctx& c, int cand, float ms)
+{
+    const int ncfg = static_cast<int>(c.tune_best.size());
+    c.tune_best[static_cast<size_t>(cand)] = std::min(c.tune_best[static_cast<size_t>(cand)], ms);
+    if (++c.tune_done == TUNE_ROUNDS * ncfg)
+        c.push_cfg = static_cast<int>(std::min_element(c.tune_best.begin(), c.tune_best.end()) - c.tune_best.begin());
 }
 
 // One Poisson iteration; updates the buffer bookkeeping.
 void poisson_iteration(Ctx& c, int mode)
 {
-    launch_push(c, mode, c.push_cfg);
+    launch_push(c, mode, c.seg_cfg);
     if (c.mixed && mode != lbm::H_STREAMED) c.pb = 1 - c.pb;
     c.hcur = 1 - c.hcur;
 }
@@ -496,27 +502,29 @@ void clear_graphs(Ctx& c)
     c.acc_graphs.clear();
 }
 
-// check_interval iterations, the first in `mode`, the rest H_STREAMED:
-// eagerly, or by replaying a captured graph of exactly these launches
-// (buildPoissonGraphSegment, GPU/InamuroCUDA.cu:1482-1534).  Kernel nodes only.
-void poisson_segment(Ctx& c, int mode)
+// check_interval iterations, the first in `mode`, the rest H_STREAMED, in
+// launch shape c.seg_cfg: eagerly, or by replaying a captured graph of exactly
+// these launches (buildPoissonGraphSegment, GPU/InamuroCUDA.cu:1482-1534).
+// Kernel nodes only.  Returns true when it had to capture the graph first.
+bool poisson_segment(Ctx& c, int mode)
 {
     if (!c.use_graphs) {
         for (int k = 0; k < c.check_interval; ++k) {
             poisson_iteration(c, mode);
             mode = lbm::H_STREAMED;
         }
-        return;
+        return false;
     }
     for (const auto& gph : c.graphs) {
-        if (gph.mode == mode && gph.hcur == c.hcur && gph.pb == c.pb && gph.ci == c.check_interval) {
+        if (gph.mode == mode && gph.hcur == c.hcur && gph.pb == c.pb && gph.ci == c.check_interval &&
+            gph.cfg == c.seg_cfg) {
             HIPCHK(hipGraphLaunch(gph.exec, c.stream));
             c.hcur = gph.hcur_after;
             c.pb = gph.pb_after;
-            return;
+            return false;
         }
     }
-    Ctx::SegGraph gph{mode, c.hcur, c.pb, c.check_interval, nullptr, 0, 0};
+    Ctx::SegGraph gph{mode, c.hcur, c.pb, c.check_interval, c.seg_cfg, nullptr, 0, 0};
     HIPCHK(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     hipGraph_t graph = nullptr;
     try {
@@ -545,6 +553,7 @@ void poisson_segment(Ctx& c, int mode)
     gph.pb_after = c.pb;
     c.graphs.push_back(gph);
     HIPCHK(hipGraphLaunch(gph.exec, c.stream));
+    return true;
 }
 
 // getp + gauge + relative change (+ dual residual) after a segment.
@@ -791,7 +800,6 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
     if (pull) {
         accel_begin(c);
     } else {
-        choose_push_config(c);
         if (acc && c.accel.source_aware_hh_init) {
             // sourceAwareHHInitKernel (GPU/InamuroCUDA.cu:2162-2172): an
             // explicit post-boundary h, read as uploaded h is.
@@ -808,10 +816,14 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
         if (iter > std::numeric_limits<uint64_t>::max() - static_cast<uint64_t>(c.check_interval))
             throw ApiError{LBM_ERR_OVERFLOW, "Poisson iteration counter overflow"};
         iter += static_cast<uint64_t>(c.check_interval);
+        bool timed = false, captured = false;
+        if (!pull) c.seg_cfg = next_segment_config(c, hmode, &timed);
+        if (timed) HIPCHK(hipEventRecord(c.ev[10], s));
         if (pull)
             accel_segment(c);
         else
-            poisson_segment(c, hmode);
+            captured = poisson_segment(c, hmode);
+        if (timed) HIPCHK(hipEventRecord(c.ev[11], s));
         HIPCHK(hipGetLastError());
         record(c, 5);
         if (pull) {
@@ -837,6 +849,7 @@ void step_one(Ctx& c, const CheckHook* hook = nullptr)
             c.perf.check_ms += elapsed(c.ev[5], c.ev[6]);
         }
         first_segment = false;
+        if (timed && !captured) tune_record(c, c.seg_cfg, elapsed(c.ev[10], c.ev[11]));
         c.perf.poisson_iter_launches += static_cast<uint64_t>(c.check_interval);
         c.perf.poisson_checks += 1;
         const DevState& st = *c.hst;
@@ -1070,6 +1083,12 @@ struct AdjWork {
     double* cg[2] = {nullptr, nullptr};
     lbm::AdjState* as = nullptr;
     lbm::AdjState* has = nullptr;
+    // Timing of the adjoint kernel's two block sizes (results do not depend
+    // on it): best time of each over the sampled reverse iterations.
+    int bs_samples = 0;
+    float bs_best[2] = {3.0e38f, 3.0e38f};
+    int rec_samples = 0;
+    std::vector<float> rec_best;
 
     ~AdjWork()
     {
@@ -1204,8 +1223,25 @@ void adj_forward_block(Ctx& c, AdjWork& a, const double* p_start, double* start,
     }
     for (uint64_t k = a0 + 1; k <= a1; ++k) {
         double* out = a.slot(c, k - a0);
-        lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, plast, D<double>(c.tau), D<double>(c.src), c.dst,
-                               c.push_cfg);
+        // Launch shape: a requested one, else the recomputation's own, timed
+        // on its first streamed iterations (the same values whichever runs).
+        int cfg = c.push_request >= 0 ? c.push_request : (c.rec_cfg >= 0 ? c.rec_cfg : 0);
+        const bool sample = c.push_request < 0 && c.rec_cfg < 0 && mode == lbm::H_STREAMED;
+        if (sample) {
+            if (a.rec_best.empty()) a.rec_best.assign(lbm::f64::PUSH_CONFIGS, 3.0e38f);
+            cfg = a.rec_samples % lbm::f64::PUSH_CONFIGS;
+            HIPCHK(hipEventRecord(c.ev[10], c.stream));
+        }
+        lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, plast, D<double>(c.tau), D<double>(c.src), c.dst, cfg);
+        if (sample) {
+            HIPCHK(hipEventRecord(c.ev[11], c.stream));
+            HIPCHK(hipEventSynchronize(c.ev[11]));
+            float& best = a.rec_best[static_cast<size_t>(cfg)];
+            best = std::min(best, elapsed(c.ev[10], c.ev[11]));
+            if (++a.rec_samples == TUNE_ROUNDS * lbm::f64::PUSH_CONFIGS)
+                c.rec_cfg = static_cast<int>(std::min_element(a.rec_best.begin(), a.rec_best.end()) -
+                                             a.rec_best.begin());
+        }
         if (k % ci == 0) {
             double* pk = a.pstore + ((k - a0) / ci) * n;
             lbm::f64::gauge(c.stream, c.g, c.ph, out, pk, c.p_prev, c.partials, c.dst);
@@ -1244,8 +1280,9 @@ void adj_forward_snaps(Ctx& c, AdjWork& a, const double* p_start, const double* 
         const bool mark = k == marks[m].first;
         const SnapView v = snap_view(c, mark ? marks[m].second : a.slot(c, 1));
         double* out = mark ? v.h : a.slot(c, 1 + (k & 1));
+        // a ping-pong pair, like the forward solve: its launch shape
         lbm::f64::poisson_iter(c.stream, mode, c.g, in, out, plast, D<double>(c.tau), D<double>(c.src), c.dst,
-                               c.push_cfg);
+                               c.push_request >= 0 ? c.push_request : (c.push_cfg >= 0 ? c.push_cfg : 0));
         if (k % ci == 0) {
             double* pk = mark ? v.p : a.pstore + ((k / ci) & 1) * n;
             lbm::f64::gauge(c.stream, c.g, c.ph, out, pk, c.p_prev, c.partials, c.dst);
@@ -1291,10 +1328,13 @@ struct StepRev {
     lbm_adjoint_stats_t* stats;
 };
 
-// Threads per block of the adjoint Poisson kernel: it streams like the
-// forward push kernel (the same loads and x-shifted stores, in reverse), so it
-// follows the forward kernel's timed choice, 768 threads or 256.
-int adj_block_size(const Ctx& c) { return c.push_cfg == 2 ? 768 : lbm::BLOCK; }
+// Threads per block of the adjoint Poisson kernel, 256 or 768; which is
+// faster depends on the grid's shape (as for the forward push kernel), so the
+// first reverse iterations of a context's first adjoint time both (three
+// interleaved samples each, every sample a plain streamed iteration that would
+// run anyway) and keep the faster.  The results do not depend on it.
+constexpr int ADJ_BS[2] = {lbm::BLOCK, 768};
+constexpr int ADJ_BS_SAMPLES = 6;
 
 // Block j of the solve (iterations jL+1 .. min(nit, (j+1)L)) from snapshot
 // `start` of check jL (nullptr for j = 0): recompute it recording every
@@ -1330,9 +1370,19 @@ void adj_block(Ctx& c, AdjWork& a, const StepRev& r, uint64_t j, double* start)
             pin = k - 1 == a0 ? sv.p : a.pstore + ((k - 1 - a0) / ci) * n;
             shift = k - 1 == a0 ? sv.shift : a.shifts + (k - 1 - a0) / ci;
         }
+        int& bs = c.adj_bs[c.adj_state_f32 ? 1 : 0];
+        const bool sample = bs < 0 && src == lbm::ADJ_SRC_PLAIN && mode == lbm::H_STREAMED;
+        const int cand = a.bs_samples % 2;
+        if (sample) HIPCHK(hipEventRecord(c.ev[10], s));
         lbm::adj::poisson(s, g, c.ph, src, mode, c.adj_state_f32, a.Lb[a.lcur], a.lpf, a.as, raw, pin,
                           shift, D<double>(c.tau), a.lS, a.ltau, k == 1 ? nullptr : a.Lb[1 - a.lcur],
-                          k == 1 ? a.lps : nullptr, a.parts, adj_block_size(c));
+                          k == 1 ? a.lps : nullptr, a.parts, sample ? ADJ_BS[cand] : (bs < 0 ? lbm::BLOCK : bs));
+        if (sample) {
+            HIPCHK(hipEventRecord(c.ev[11], s));
+            HIPCHK(hipEventSynchronize(c.ev[11]));
+            a.bs_best[cand] = std::min(a.bs_best[cand], elapsed(c.ev[10], c.ev[11]));
+            if (++a.bs_samples == ADJ_BS_SAMPLES) bs = ADJ_BS[a.bs_best[1] < a.bs_best[0] ? 1 : 0];
+        }
         if (k > 1) a.lcur = 1 - a.lcur;
         if ((k - 1) % ci == 0)  // a check interval of the solve reversed
             lbm::adj::converge(s, g, a.lS, a.lS_prev, nit - k + 1, c.tol, k + ci - 1 == nit ? 1 : 0, a.parts,
@@ -1405,8 +1455,15 @@ void adj_reverse_range(Ctx& c, AdjWork& a, const StepRev& r, uint64_t lo, uint64
 
 // Reverse of one step whose start state is st (already restored into c).
 // snaps: the forward pass's snapshots of this solve, if it recorded them.
+// reverse_solve = false for the last step: J reads phi_T, which getMacro forms
+// before that step's Poisson solve, so every adjoint entering the solve and
+// correct_uvw is zero (lambda_p, lambda_u/v/w start at zero) and their
+// transposes would only propagate zeros.  Then only the correct_uvw adjoint
+// runs (it resets the gauge adjoint state), lambda_p_start is zero, and the
+// solve is neither recomputed nor reversed.
 void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*>& snaps, int have_snaps,
-              size_t pool_free, int step, uint64_t nit, int ccur, lbm_adjoint_stats_t& stats)
+              size_t pool_free, int step, uint64_t nit, int ccur, lbm_adjoint_stats_t& stats,
+              bool reverse_solve)
 {
     hipStream_t s = c.stream;
     const Grid& g = c.g;
@@ -1427,15 +1484,22 @@ void adj_step(Ctx& c, AdjWork& a, const StepState& st, const std::vector<double*
     HIPCHK(hipMemsetAsync(a.ltau, 0, n * sizeof(double), s));
     a.lcur = 0;
     const StepRev r{p_start, nit, step, &stats};
-    if (have_snaps == SNAPS_ALL || nblocks == 1) {
-        for (uint64_t j = nblocks; j-- > 0;) adj_block(c, a, r, j, j > 0 ? snaps[j - 1] : nullptr);
-    } else if (have_snaps == SNAPS_TOP) {
-        adj_reverse_segments(c, a, r, revolve_plan(0, nblocks, a.snap_cap - pool_free), nullptr, pool_free);
+    if (!reverse_solve) {
+        lbm::adj::correct(s, g, c.ph, a.pfinal + static_cast<size_t>(step) * n, D<double>(c.rho), a.lu, a.lv,
+                          a.lw, a.lp, a.lrho, a.lpf, a.parts, a.as);
+        HIPCHK(hipMemsetAsync(a.lps, 0, n * sizeof(double), s));
+        HIPCHK(hipMemsetAsync(a.conv + 2 * static_cast<size_t>(step), 0, 2 * sizeof(double), s));
     } else {
-        adj_reverse_range(c, a, r, 0, nblocks, nullptr, pool_free);
+        if (have_snaps == SNAPS_ALL || nblocks == 1) {
+            for (uint64_t j = nblocks; j-- > 0;) adj_block(c, a, r, j, j > 0 ? snaps[j - 1] : nullptr);
+        } else if (have_snaps == SNAPS_TOP) {
+            adj_reverse_segments(c, a, r, revolve_plan(0, nblocks, a.snap_cap - pool_free), nullptr, pool_free);
+        } else {
+            adj_reverse_range(c, a, r, 0, nblocks, nullptr, pool_free);
+        }
+        lbm::adj::converge_record(s, a.as, a.conv, step);
+        stats.adjoint_iterations += nit;
     }
-    lbm::adj::converge_record(s, a.as, a.conv, step);
-    stats.adjoint_iterations += nit;
     lbm::adj::source(s, g, c.ph, D<double>(c.fei), D<double>(c.rho), a.ltau, a.lS, a.lfei, a.as, a.lrho,
                      a.lps, a.lmac, a.parts);
     lbm::adj::collide(s, first ? 1 : 0, g, c.ph, s_fei, s_rho, s_u, s_v, s_w, a.cf[ccur], a.cg[ccur], a.lmac,
@@ -1494,7 +1558,7 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
         for (int k = 0; k < T; ++k) {
             if (iters[k] == 0 || iters[k] % ci != 0)
                 throw ApiError{LBM_ERR_ARG, "scheduled iteration counts must be positive multiples of the check interval"};
-            sched_max = std::max(sched_max, iters[k]);
+            if (k + 1 < T) sched_max = std::max(sched_max, iters[k]);  // the solves the reverse sweep replays
         }
     hipStream_t s = c.stream;
     const Grid& g = c.g;
@@ -1608,7 +1672,8 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
         const size_t pool_mark = a.snap_next;
         a.pool_mark[ks] = pool_mark;
         SnapRecorder rec{&a, &a.step_snaps[ks]};
-        if (iters) {
+        const bool reversed = k + 1 < T;  // the last step's solve is not reversed (adj_step)
+        if (iters && reversed) {
             // a scheduled solve whose block snapshots do not fit keeps the
             // top level of its binomial checkpointing instead
             const uint64_t nb = (iters[k] + L - 1) / L;
@@ -1619,11 +1684,14 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
             }
         }
         const CheckHook hook{record_snapshot, &rec};
-        step_one(c, &hook);
+        step_one(c, reversed ? &hook : nullptr);
         const uint64_t nit = c.history.back();
         report(c, LBM_PROGRESS_FORWARD_STEP, k, nit);
         const size_t need = static_cast<size_t>((nit + L - 1) / L - 1);
-        if (!rec.marks.empty()) {
+        if (!reversed) {
+            a.step_snaps[ks].clear();
+            a.snap_next = pool_mark;
+        } else if (!rec.marks.empty()) {
             a.snap_next = pool_mark + rec.marks.size();
             a.have_snaps[ks] = SNAPS_TOP;
         } else if (rec.overflow) {
@@ -1669,7 +1737,8 @@ void run_adjoint(Ctx& c, const double* phi0, const double* target, int T, int K,
             const size_t ks = static_cast<size_t>(k);
             const StepState& st = a.seg[static_cast<size_t>(k - k0)];
             restore_state(c, st);
-            adj_step(c, a, st, a.step_snaps[ks], a.have_snaps[ks], a.pool_mark[ks], k, sched[ks], ccur, stats);
+            adj_step(c, a, st, a.step_snaps[ks], a.have_snaps[ks], a.pool_mark[ks], k, sched[ks], ccur, stats,
+                     k + 1 < T);
             report(c, LBM_PROGRESS_REVERSE_STEP, k, sched[ks]);
             ccur = 1 - ccur;
         }
@@ -1979,7 +2048,10 @@ int lbm_set_poisson_launch(void* ctx, int config)
             throw ApiError{LBM_ERR_ARG, "Poisson launch config must be -1 (automatic) or 0.." +
                                             std::to_string(ncfg - 1) + " for this precision"};
         c.push_request = config;
-        c.push_cfg = -1;
+        c.push_cfg = config;  // -1: tune again
+        c.tune_best.clear();
+        c.tune_done = 0;
+        c.rec_cfg = -1;
         clear_graphs(c);
     });
 }

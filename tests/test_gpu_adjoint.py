@@ -65,7 +65,10 @@ def test_adjoint_matches_oracle(rho_L, ci, steps, cx, lx):
     assert abs(gk - gko) <= ADJ_TOL * abs(gko), (gk, gko)
     st = sim.adjoint_stats()
     assert st.steps == steps and st.forward_iterations == sum(iters)
-    assert st.adjoint_iterations == sum(iters)
+    # the last step's solve carries no adjoint (J reads phi_T, formed before
+    # it) and is neither recomputed nor reversed; the gradient above still
+    # matches the oracle, which reverses it
+    assert st.adjoint_iterations == sum(iters[:-1])
     # the reverse solves' convergence record is a relative change and an
     # iteration count within the solves
     assert 0.0 <= st.final_relative_change <= 1.0
@@ -100,7 +103,7 @@ def test_recomputation_does_not_change_the_result():
         J, g, k = sim.adjoint(phi0, target, 4, checkpoint_interval=K)
         st = sim.adjoint_stats()
         if L:
-            assert st.store_slots < m and st.recomputed_iterations >= sum(iters), (st.store_slots, m)
+            assert st.store_slots < m and st.recomputed_iterations >= sum(iters[:-1]), (st.store_slots, m)
         assert J == J0 and k == k0 and np.array_equal(g, g0), (K, L)
     # Given the iteration counts, the budget split is chosen for the longest
     # solve, and a solve whose block snapshots do not fit keeps the top level
@@ -270,9 +273,10 @@ def test_set_poisson_validates_every_argument_first():
 
 
 def test_launch_shape_does_not_change_the_gradient():
-    """The forward kernel's launch shape (which the recomputation and the
-    adjoint Poisson kernel follow: 256- or 768-thread blocks) changes no bit
-    of J or the gradients."""
+    """The forward kernel's launch shape (which the recomputation follows)
+    and the adjoint Poisson kernel's block size (256 or 768 threads: a
+    context's first adjoint call times both on its first reverse iterations,
+    later calls use the faster) change no bit of J or the gradients."""
     P = params(lx=16, ly=12, lz=12, rho_L=1000.0, DD=8.0)
     phi0, target, _ = case(P, seed=5)
     out = []
@@ -280,6 +284,7 @@ def test_launch_shape_does_not_change_the_gradient():
         announce(f"adjoint with launch shape {cfg}")
         sim = hip.InamuroHIP(P)
         sim.set_poisson_launch(cfg)
+        out.append(sim.adjoint(phi0, target, 3, checkpoint_interval=2))
         out.append(sim.adjoint(phi0, target, 3, checkpoint_interval=2))
         assert sim.poisson_launch() == cfg
         sim.close()
