@@ -328,7 +328,8 @@ Ctx* create(const lbm_params_t& P, int precision)
         c->p_prev = dalloc(n);
         if (c->mixed) c->scratch1 = dalloc(n);
         const size_t nparts = std::max<size_t>(
-            std::max<size_t>(2 * static_cast<size_t>(lbm::grid_blocks(c->g.n)),
+            std::max<size_t>(std::max<size_t>(2 * static_cast<size_t>(lbm::grid_blocks(c->g.n)),
+                                              2 * static_cast<size_t>(lbm::acc::pull_blocks(c->g))),
                              lbm::SOURCE_TERMS * static_cast<size_t>(lbm::stencil_blocks(c->g))),
             9 * static_cast<size_t>(lbm::RED_BLOCKS));
         c->partials = dalloc(nparts);

@@ -200,7 +200,10 @@ namespace acc {
 // onepass adds the fixed-point relaxation omega.  anderson (onepass only):
 // 0 none, 1 first iteration (also writes the history pr = hn - h, pi = hn),
 // 2 later iterations (block partials of the Anderson dot products into
-// parts, 2 * grid_blocks(g.n) doubles; pn is left to anderson_mix).
+// parts, 2 * pull_blocks(g) doubles; pn is left to anderson_mix).
+// Blocks of the staged pull and scalar kernels (tiles of the collision
+// kernels, marching a grid-dependent number of layers).
+int pull_blocks(const Grid& g);
 void pull(hipStream_t s, const Grid& g, int onepass, int anderson, double scale, double omega,
           const double* h, const double* p, const double* tau, const double* src, double* hn,
           double* pn, double* pr, double* pi, double* parts);

@@ -29,70 +29,142 @@ __device__ __forceinline__ double reduce_parts(const double* __restrict__ parts,
     return block_sum(acc, lds);
 }
 
+// p, tau_h and S of one z layer and its one-cell x/y halo, staged in LDS in
+// a ring of three layers (the collision kernels' tile: STENCIL_TX x
+// STENCIL_TY cells marching zc layers; see pull_zchunk).  The x halo is the
+// stencils' mirror (x = -1 -> 1, x = lx -> lx-2); y and z wrap.  Loads of the
+// next layer go to registers before the current layer is computed and land
+// in LDS after it.
+template <int NF>
+struct AccStage {
+    static constexpr int PER = (STENCIL_PLANE + BLOCK - 1) / BLOCK;
+    double next[NF][PER];
+    __device__ __forceinline__ void fetch(const double* const (&src)[NF], int x0, int y0, int gz, const Grid& g)
+    {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = threadIdx.x + j * BLOCK;
+            if (e < STENCIL_PLANE) {
+                const int64_t i = stencil_source(e, x0, y0, gz, g);
+#pragma unroll
+                for (int f = 0; f < NF; ++f) next[f][j] = src[f][i];
+            }
+        }
+    }
+    __device__ __forceinline__ void land(double (&staged)[NF][3][STENCIL_PLANE], int slot)
+    {
+#pragma unroll
+        for (int j = 0; j < PER; ++j) {
+            const int e = threadIdx.x + j * BLOCK;
+            if (e < STENCIL_PLANE) {
+#pragma unroll
+                for (int f = 0; f < NF; ++f) staged[f][slot][e] = next[f][j];
+            }
+        }
+    }
+};
+
+// LDS slot (ring slot times plane, plus place in the plane) of the source
+// x - c_q of tile cell (tx, ty) in the layer whose ring slots are sm, s0, sp
+// (z - 1, z, z + 1); cx is the x offset actually taken (c_x of q, or -c_x
+// for the mirror image).
+__device__ __forceinline__ int pull_slot(int q, int cx, int tx, int ty, int sm, int s0, int sp)
+{
+    const int ring = CZ[q] == 1 ? sm : (CZ[q] == -1 ? sp : s0);
+    return ring * STENCIL_PLANE + (tx + 1 - cx) + STENCIL_SX * (ty + 1 - CY[q]);
+}
+
+// Slip partner of direction q (the same c_y, c_z and weight, c_x reversed).
+__device__ __forceinline__ constexpr int slip_partner(int q)
+{
+    return q == 1 ? 4 : q == 4 ? 1 : q == 7 ? 8 : q == 8 ? 7 : q == 9 ? 14 : q == 14 ? 9
+         : q == 10 ? 13 : q == 13 ? 10 : q == 12 ? 11 : q == 11 ? 12 : q;
+}
+
 // One iteration of the pull form.  ONEPASS: fixed-point relaxation against
 // the cell's own h.  AND (onepass only): 0 none; 1 the first Anderson
 // iteration, which also stores the history (residual image - h, image);
 // 2 later iterations, which add the block partials of <R_prev, R - R_prev>
-// and |R - R_prev|^2 and leave p to the mixing kernel.
+// and |R - R_prev|^2 and leave p to the mixing kernel.  The 15 source cells'
+// p, tau_h and S come from LDS; the source populations from HBM.  The x-slip
+// rule is folded into each direction: at x = 0 (x = lx-1) a direction whose
+// source lies beyond the wall takes the image of its slip partner, i.e. the
+// partner's population pulled from the mirror cell x + c_x; the image the
+// reference's periodic pull forms for it is overwritten, so it is not formed.
+// Each direction is then independent, and every access is a per-direction
+// base (uniform across the block) plus a 32-bit cell index (grids hold at
+// most 2^31 - 1 cells), which keeps the register count low.
 template <int ONEPASS, int AND>
 __global__ __launch_bounds__(BLOCK) void k_acc_pull(
-    Grid g, double scale, double omega, const double* __restrict__ h, const double* __restrict__ p,
+    Grid g, int zc, double scale, double omega, const double* __restrict__ h, const double* __restrict__ p,
     const double* __restrict__ tau, const double* __restrict__ src, double* __restrict__ hn,
     double* __restrict__ pn, double* __restrict__ pr, double* __restrict__ pi,
     double* __restrict__ parts)
 {
     __shared__ double lds[4];
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
+    __shared__ double staged[3][3][STENCIL_PLANE];  // p, tau_h, S; ring of 3 layers
+    const int ntx = (g.lx + STENCIL_TX - 1) / STENCIL_TX, nty = (g.ly + STENCIL_TY - 1) / STENCIL_TY;
+    const int b = static_cast<int>(blockIdx.x);
+    const int x0 = (b % ntx) * STENCIL_TX, y0 = ((b / ntx) % nty) * STENCIL_TY;
+    const int zb = (b / (ntx * nty)) * zc, ze = min(g.lz, zb + zc);
+    const double* const fields[3] = {p, tau, src};
+    AccStage<3> st;
+    st.fetch(fields, x0, y0, wrap(zb, -1, g.lz), g);
+    st.land(staged, 0);
+    st.fetch(fields, x0, y0, zb, g);
+    st.land(staged, 1);
+    st.fetch(fields, x0, y0, wrap(zb, 1, g.lz), g);
+    const int tx = threadIdx.x % STENCIL_TX, ty = threadIdx.x / STENCIL_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < g.lx && y < g.ly;
+    const int xm = wrap(x, -1, g.lx), xp = wrap(x, 1, g.lx);
+    const int ym = wrap(y, -1, g.ly), yp = wrap(y, 1, g.ly);
     double num = 0.0, den = 0.0;
-    if (c < g.n) {
-        int x, y, z;
-        cell_xyz(c, g, x, y, z);
-        const int xm = wrap(x, -1, g.lx), xp = wrap(x, 1, g.lx);
-        const int ym = wrap(y, -1, g.ly), yp = wrap(y, 1, g.ly);
-        const int zm = wrap(z, -1, g.lz), zp = wrap(z, 1, g.lz);
-        double im[Q];
+    for (int z = zb; z < ze; ++z) {
+        const int sm = (z - zb) % 3, s0 = (z - zb + 1) % 3, sp = (z - zb + 2) % 3;
+        st.land(staged, sp);
+        __syncthreads();
+        if (z + 1 < ze) st.fetch(fields, x0, y0, wrap(z + 1, 1, g.lz), g);
+        if (valid) {
+            const int zm = wrap(z, -1, g.lz), zp = wrap(z, 1, g.lz);
+            const int64_t c = lin(x, y, z, g);
+            double p_sum = 0.0;
 #pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int sx = CX[q] == 1 ? xm : (CX[q] == -1 ? xp : x);
-            const int sy = CY[q] == 1 ? ym : (CY[q] == -1 ? yp : y);
-            const int sz = CZ[q] == 1 ? zm : (CZ[q] == -1 ? zp : z);
-            const int64_t s = lin(sx, sy, sz, g);
-            const double tauh = tau[s];
-            const double hequ = EW[q] * p[s];
-            const double h_old = h[q * g.n + s];
-            im[q] = h_old - scale * (h_old - hequ) / tauh - (EW[q] / 3.0) * src[s];
-        }
-        if (x == 0) {
-#pragma unroll
-            for (int r = 0; r < 5; ++r) im[SLIP_POS[r]] = im[SLIP_NEG[r]];
-        } else if (x == g.lx - 1) {
-#pragma unroll
-            for (int r = 0; r < 5; ++r) im[SLIP_NEG[r]] = im[SLIP_POS[r]];
-        }
-        double p_sum = 0.0;
-#pragma unroll
-        for (int q = 0; q < Q; ++q) {
-            const int64_t k = q * g.n + c;
-            double v = im[q];
-            double h_own = 0.0;
-            if (ONEPASS) {
-                h_own = h[k];
-                v = h_own + omega * (v - h_own);
+            for (int q = 0; q < Q; ++q) {
+                const bool mirrored = (x == 0 && CX[q] == 1) || (x == g.lx - 1 && CX[q] == -1);
+                const int cx = mirrored ? -CX[q] : CX[q];
+                const int qa = mirrored ? slip_partner(q) : q;
+                const int sx = cx == 1 ? xm : (cx == -1 ? xp : x);
+                const int sy = CY[q] == 1 ? ym : (CY[q] == -1 ? yp : y);
+                const int sz = CZ[q] == 1 ? zm : (CZ[q] == -1 ? zp : z);
+                const int slot = pull_slot(q, cx, tx, ty, sm, s0, sp);
+                const double tauh = staged[1][0][slot];
+                const double hequ = EW[q] * staged[0][0][slot];
+                const uint32_t src_cell = static_cast<uint32_t>((sz * g.ly + sy) * g.lx + sx);
+                const double h_old = (h + qa * g.n)[src_cell];
+                const uint32_t cc = static_cast<uint32_t>(c);
+                double v = h_old - scale * (h_old - hequ) / tauh - (EW[q] / 3.0) * staged[2][0][slot];
+                double h_own = 0.0;
+                if (ONEPASS) {
+                    h_own = (h + q * g.n)[cc];
+                    v = h_own + omega * (v - h_own);
+                }
+                (hn + q * g.n)[cc] = v;
+                p_sum += v;
+                if (AND == 1) {
+                    (pr + q * g.n)[cc] = v - h_own;
+                    (pi + q * g.n)[cc] = v;
+                } else if (AND == 2) {
+                    const double residual = v - h_own;
+                    const double prev = (pr + q * g.n)[cc];
+                    const double delta = residual - prev;
+                    num += prev * delta;
+                    den += delta * delta;
+                }
             }
-            hn[k] = v;
-            p_sum += v;
-            if (AND == 1) {
-                pr[k] = v - h_own;
-                pi[k] = v;
-            } else if (AND == 2) {
-                const double residual = v - h_own;
-                const double prev = pr[k];
-                const double delta = residual - prev;
-                num += prev * delta;
-                den += delta * delta;
-            }
+            if (AND != 2) pn[c] = p_sum;
         }
-        if (AND != 2) pn[c] = p_sum;
+        __syncthreads();  // the ring slot of layer z - 1 is overwritten next
     }
     if (AND == 2) {
         const double sn = block_sum(num, lds);
@@ -150,30 +222,45 @@ __global__ __launch_bounds__(BLOCK) void k_acc_apply(Grid g, const DevState* __r
 }
 
 // scalarPressureJacobiKernel (GPU/InamuroCUDA.cu:848-908): the 14 moving
-// neighbours x - c_q in q order.  pressure_bounce_direction replaces a
-// direction whose source lies beyond an x wall (c_x = +1 at x = 0, c_x = -1
-// at x = lx-1) by its slip partner, which differs only in c_x, so the
-// neighbour is the mirror image x + c_x.
-__global__ __launch_bounds__(BLOCK) void k_acc_scalar(Grid g, double scale, const double* __restrict__ p,
+// neighbours x - c_q in q order, p staged in LDS like k_acc_pull's fields.
+// pressure_bounce_direction replaces a direction whose source lies beyond an
+// x wall by its slip partner, which differs only in c_x, so the neighbour is
+// the mirror image: exactly what the staged halo holds there.
+__global__ __launch_bounds__(BLOCK) void k_acc_scalar(Grid g, int zc, double scale, const double* __restrict__ p,
                                                       const double* __restrict__ rho,
                                                       const double* __restrict__ src,
                                                       double* __restrict__ pn)
 {
-    const int64_t c = static_cast<int64_t>(blockIdx.x) * BLOCK + threadIdx.x;
-    if (c >= g.n) return;
-    int x, y, z;
-    cell_xyz(c, g, x, y, z);
-    double neighbor_sum = 0.0;
+    __shared__ double staged[1][3][STENCIL_PLANE];
+    const int ntx = (g.lx + STENCIL_TX - 1) / STENCIL_TX, nty = (g.ly + STENCIL_TY - 1) / STENCIL_TY;
+    const int b = static_cast<int>(blockIdx.x);
+    const int x0 = (b % ntx) * STENCIL_TX, y0 = ((b / ntx) % nty) * STENCIL_TY;
+    const int zb = (b / (ntx * nty)) * zc, ze = min(g.lz, zb + zc);
+    const double* const fields[1] = {p};
+    AccStage<1> st;
+    st.fetch(fields, x0, y0, wrap(zb, -1, g.lz), g);
+    st.land(staged, 0);
+    st.fetch(fields, x0, y0, zb, g);
+    st.land(staged, 1);
+    st.fetch(fields, x0, y0, wrap(zb, 1, g.lz), g);
+    const int tx = threadIdx.x % STENCIL_TX, ty = threadIdx.x / STENCIL_TX;
+    const int x = x0 + tx, y = y0 + ty;
+    const bool valid = x < g.lx && y < g.ly;
+    for (int z = zb; z < ze; ++z) {
+        const int sm = (z - zb) % 3, s0 = (z - zb + 1) % 3, sp = (z - zb + 2) % 3;
+        st.land(staged, sp);
+        __syncthreads();
+        if (z + 1 < ze) st.fetch(fields, x0, y0, wrap(z + 1, 1, g.lz), g);
+        if (valid) {
+            const int64_t c = lin(x, y, z, g);
+            double neighbor_sum = 0.0;
 #pragma unroll
-    for (int q = 1; q < Q; ++q) {
-        const bool mirrored = (x == 0 && CX[q] == 1) || (x == g.lx - 1 && CX[q] == -1);
-        const int sx = wrap(x, mirrored ? CX[q] : -CX[q], g.lx);
-        const int sy = wrap(y, -CY[q], g.ly);
-        const int sz = wrap(z, -CZ[q], g.lz);
-        neighbor_sum += p[lin(sx, sy, sz, g)];
+            for (int q = 1; q < Q; ++q) neighbor_sum += staged[0][0][pull_slot(q, CX[q], tx, ty, sm, s0, sp)];
+            const double rhs = scale * rho[c] * src[c];
+            pn[c] = (neighbor_sum - 5.0 * rhs) / 14.0;
+        }
+        __syncthreads();
     }
-    const double rhs = scale * rho[c] * src[c];
-    pn[c] = (neighbor_sum - 5.0 * rhs) / 14.0;
 }
 
 // sourceAwareHHInitKernel (GPU/InamuroCUDA.cu:905-928).
@@ -310,32 +397,51 @@ __global__ __launch_bounds__(BLOCK) void k_acc_two_grid(Grid g, int nbx, int nby
 
 namespace acc {
 
+// Layers a block of the staged kernels marches: up to STENCIL_ZCHUNK, fewer
+// on small grids so that the launch still has about four blocks per CU
+// (1024); the staging of the two extra layers is then spread over fewer
+// computed ones.
+int pull_zchunk(const Grid& g)
+{
+    const int64_t tiles = static_cast<int64_t>((g.lx + STENCIL_TX - 1) / STENCIL_TX) *
+                          ((g.ly + STENCIL_TY - 1) / STENCIL_TY);
+    const int64_t zc = tiles * g.lz / 1024;
+    return static_cast<int>(zc < 1 ? 1 : (zc > STENCIL_ZCHUNK ? STENCIL_ZCHUNK : zc));
+}
+
+int pull_blocks(const Grid& g)
+{
+    const int zc = pull_zchunk(g);
+    return ((g.lx + STENCIL_TX - 1) / STENCIL_TX) * ((g.ly + STENCIL_TY - 1) / STENCIL_TY) *
+           ((g.lz + zc - 1) / zc);
+}
+
 void pull(hipStream_t s, const Grid& g, int onepass, int anderson, double scale, double omega,
           const double* h, const double* p, const double* tau, const double* src, double* hn,
           double* pn, double* pr, double* pi, double* parts)
 {
-    const int nb = grid_blocks(g.n);
+    const int nb = pull_blocks(g), zc = pull_zchunk(g);
     if (!onepass)
-        k_acc_pull<0, 0><<<nb, BLOCK, 0, s>>>(g, scale, omega, h, p, tau, src, hn, pn, pr, pi, parts);
+        k_acc_pull<0, 0><<<nb, BLOCK, 0, s>>>(g, zc, scale, omega, h, p, tau, src, hn, pn, pr, pi, parts);
     else if (anderson == 0)
-        k_acc_pull<1, 0><<<nb, BLOCK, 0, s>>>(g, scale, omega, h, p, tau, src, hn, pn, pr, pi, parts);
+        k_acc_pull<1, 0><<<nb, BLOCK, 0, s>>>(g, zc, scale, omega, h, p, tau, src, hn, pn, pr, pi, parts);
     else if (anderson == 1)
-        k_acc_pull<1, 1><<<nb, BLOCK, 0, s>>>(g, scale, omega, h, p, tau, src, hn, pn, pr, pi, parts);
+        k_acc_pull<1, 1><<<nb, BLOCK, 0, s>>>(g, zc, scale, omega, h, p, tau, src, hn, pn, pr, pi, parts);
     else
-        k_acc_pull<1, 2><<<nb, BLOCK, 0, s>>>(g, scale, omega, h, p, tau, src, hn, pn, pr, pi, parts);
+        k_acc_pull<1, 2><<<nb, BLOCK, 0, s>>>(g, zc, scale, omega, h, p, tau, src, hn, pn, pr, pi, parts);
 }
 
 void anderson_mix(hipStream_t s, const Grid& g, double beta_max, const double* parts, DevState* st,
                   const double* h, const double* img, double* pi_to_a, double* pr, double* pn)
 {
-    k_acc_alpha<<<1, BLOCK, 0, s>>>(parts, grid_blocks(g.n), beta_max, st);
+    k_acc_alpha<<<1, BLOCK, 0, s>>>(parts, pull_blocks(g), beta_max, st);
     k_acc_apply<<<grid_blocks(g.n), BLOCK, 0, s>>>(g, st, h, img, pi_to_a, pr, pn);
 }
 
 void scalar(hipStream_t s, const Grid& g, double scale, const double* p, const double* rho,
             const double* src, double* pn)
 {
-    k_acc_scalar<<<grid_blocks(g.n), BLOCK, 0, s>>>(g, scale, p, rho, src, pn);
+    k_acc_scalar<<<pull_blocks(g), BLOCK, 0, s>>>(g, pull_zchunk(g), scale, p, rho, src, pn);
 }
 
 void source_init(hipStream_t s, const Grid& g, double scale, const double* p, const double* tau,
