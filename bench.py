@@ -530,6 +530,9 @@ def main(argv=None):
     ap.add_argument("--accel", action="store_true",
                     help="the reference's Poisson accelerators (FP64): iterations to tolerance on the 64^3 "
                          "droplet and time per iteration at 256^3; prints one JSON line and exits")
+    ap.add_argument("--accel-probe", action="store_true",
+                    help="the pull-form accelerators (onepass, onepass + Anderson(1), scalar) at 256^3 with the "
+                         "Poisson counts frozen at [100, 300] (for rocprofv3 runs) and exit")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run one short solve per precision (for rocprofv3 --pmc runs) and exit")
     ap.add_argument("--adjoint-probe", action="store_true",
@@ -548,6 +551,18 @@ def main(argv=None):
 
     if args.accel:
         print(json.dumps(run_accel(hip)), flush=True)
+        return 0
+    if args.accel_probe:
+        for name, accel, _ in ACCEL_VARIANTS:
+            if accel.get("variant") not in ("onepass", "scalar") or accel.get("two_grid"):
+                continue
+            sim, _ = make_sim(hip, WORKLOADS["static_droplet_256"], "fp64")
+            sim.set_poisson_accel(**accel)
+            sim.set_iteration_schedule([100, 300])
+            sim.performTimeStepGPU(2)
+            sim.synchronize()
+            sim.close()
+            print(f"accel probe {name}: 2 steps, 400 iterations", flush=True)
         return 0
     if args.adjoint_probe:
         rec = run_adjoint_falling(hip, 2, 2, "fp32", iterations=[100, 300])
