@@ -101,6 +101,32 @@ def test_walls_and_gravity_on_ragged_grids(name):
     run_pair(name, walls=True)
 
 
+@pytest.mark.parametrize("name", ["70x9x13", "3x66x21"])
+def test_dual_residual_on_ragged_grids(name):
+    """The kinetic fixed-point residual (k_fixed_point's one-cell neighbour
+    reads) with the dual exit, FP64, against the oracle."""
+    P = params(name)
+    print(f"ragged {name}: dual residual, 2 steps", flush=True)
+    o = oracle(name, P)
+    start = o.fields()
+    o.set_poisson(dual=True)
+    sim = gpu(name, P, start)
+    sim.setPoisson(100, 1e-3, 0, True)
+    for step in range(2):
+        oi, gi = o.step(1), sim.step(1)
+        assert gi == oi, (step + 1, oi, gi)
+        gd, od = sim.getLastPoissonDiagnostics(), o.diag()
+        assert gd.fixed_point_evaluated == 1 and gd.converged == 1
+        for k in ("fixed_point_h_relative", "fixed_point_p_relative", "fixed_point_relative"):
+            g_, o_ = getattr(gd, k), getattr(od, k)
+            assert abs(g_ - o_) <= 1e-8 * abs(o_) + 1e-14, (k, g_, o_)
+    ref, a = o.fields(), sim.downloadFieldsToCPU()
+    for k in FIELDS:
+        assert rel(a[k], ref[k]) <= 1e-12, (k, rel(a[k], ref[k]))
+    sim.close()
+    o.close()
+
+
 ACCEL = {
     "anderson_two_grid_spatial": dict(variant="onepass", anderson_m1=1, two_grid=1, spatial_diagnostics=1),
     "onepass_w0.8_relax_source_aware": dict(variant="onepass", fixed_point_relax=0.8,
