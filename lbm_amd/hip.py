@@ -333,9 +333,12 @@ class InamuroHIP:
 
     # -- configuration ------------------------------------------------------
     def setPoisson(self, check_interval=100, tolerance=1e-3, iteration_limit=0, dual_residual=False):
+        """dual_residual: False/0 off, True/1 the dual exit, 2 the kinetic
+        residual evaluated without changing the exit (the fixed-point shadow)."""
+        dual = int(dual_residual)
         self._check(load_library().lbm_set_poisson(self._h, int(check_interval), float(tolerance),
-                                                   int(iteration_limit), int(bool(dual_residual))))
-        self._poisson = (int(check_interval), float(tolerance), int(iteration_limit), bool(dual_residual))
+                                                   int(iteration_limit), dual))
+        self._poisson = (int(check_interval), float(tolerance), int(iteration_limit), dual)
 
     def _cfg(self):
         return getattr(self, "_poisson", (100, 1e-3, 0, False))
@@ -349,8 +352,20 @@ class InamuroHIP:
         self.setPoisson(ci, tol, max_iterations, dual)
 
     def setUsePoissonDualResidual(self, enabled: bool):
-        ci, tol, lim, _ = self._cfg()
-        self.setPoisson(ci, tol, lim, enabled)
+        ci, tol, lim, dual = self._cfg()
+        self.setPoisson(ci, tol, lim, 1 if enabled else (0 if dual == 1 else dual))
+
+    def setEnablePoissonFixedPointShadow(self, enabled: bool):
+        """GPU/InamuroCUDA.cu:2568-2571: the kinetic residual at every check,
+        reported in the diagnostics, without changing the exit."""
+        ci, tol, lim, dual = self._cfg()
+        if dual != 1:
+            self.setPoisson(ci, tol, lim, 2 if enabled else 0)
+
+    def setUsePoissonDeterministicReductions(self, enabled: bool):
+        """GPU/InamuroCUDA.cu:2563-2566.  Every reduction of this build is a
+        fixed-shape tree and deterministic whatever the flag; accepted for the
+        reference's interface."""
 
     def set_poisson_launch(self, config: int = -1):
         """Launch shape of the Poisson kernel: -1 chooses by timing at the
@@ -512,6 +527,16 @@ class InamuroHIP:
 
     def reset_perf(self):
         self._check(load_library().lbm_perf_reset(self._h))
+
+    def setEnablePoissonDetailTiming(self, enabled: bool):
+        """GPU/InamuroCUDA.cu:2558-2561: one fused kernel per iteration here,
+        so the detail is the iteration and check totals of perf(); turns
+        event timing on."""
+        if enabled:
+            self.set_timing(True)
+
+    def resetPerformanceMetrics(self):
+        self.reset_perf()
 
     # -- discrete adjoint (no counterpart in the reference) ----------------------
     def initial_rho(self, phi0):

@@ -40,6 +40,7 @@ InamuroHIP::InamuroHIP(const lbm_params_t& params, int precision)
 {
     check(lbm_create(&params, precision, &ctx), nullptr);
     n = static_cast<std::size_t>(params.lx) * params.ly * params.lz;
+    precision_ = precision;
     lbm_poisson_accel_defaults(&accel_);
 }
 
@@ -48,7 +49,17 @@ InamuroHIP::~InamuroHIP()
     if (ctx) lbm_destroy(ctx);
 }
 
-void InamuroHIP::performTimeStepGPU() { check(lbm_step(ctx, 1), ctx); }
+void InamuroHIP::performTimeStepGPU()
+{
+    // GPU/InamuroCUDA.cu:1857-1865 (the accelerator refusal just before it
+    // is the library's own).
+    if (dual_ == 1 && (interval_ != 100 || tol_ != 1.0e-3 || limit_ != 0 || !deterministic_ ||
+                       accel_.spatial_diagnostics))
+        throw std::runtime_error(
+            "dual-residual exit requires check_interval=100, tolerance=1e-3, poisson_iteration_limit=0, "
+            "deterministic reductions, no spatial diagnostics");
+    check(lbm_step(ctx, 1), ctx);
+}
 void InamuroHIP::synchronize() const { check(lbm_synchronize(ctx), ctx); }
 
 void InamuroHIP::apply() { check(lbm_set_poisson(ctx, interval_, tol_, limit_, dual_), ctx); }
@@ -84,6 +95,7 @@ void InamuroHIP::setUsePoissonDualResidual(bool enabled)
 {
     dual_ = enabled ? 1 : (dual_ == 1 ? 0 : dual_);
     apply();
+    if (enabled) deterministic_ = true;  // GPU/InamuroCUDA.cu:2573-2580
 }
 
 void InamuroHIP::setEnablePoissonFixedPointShadow(bool enabled)
@@ -95,6 +107,98 @@ void InamuroHIP::setEnablePoissonFixedPointShadow(bool enabled)
 void InamuroHIP::setUsePoissonGraph(bool enabled)
 {
     check(lbm_set_poisson_graph(ctx, enabled ? 1 : 0), ctx);
+    graph_ = enabled;
+}
+
+void InamuroHIP::setEnablePoissonDetailTiming(bool enabled)
+{
+    if (enabled) setTiming(true);
+    detail_ = enabled;
+}
+
+void InamuroHIP::setUsePoissonDeterministicReductions(bool enabled) { deterministic_ = enabled; }
+
+void InamuroHIP::resetPerformanceMetrics() { check(lbm_perf_reset(ctx), ctx); }
+
+namespace {
+
+const char* variant_name(int v)
+{
+    return v == LBM_POISSON_SCALAR ? "scalar" : v == LBM_POISSON_ONEPASS ? "onepass"
+         : v == LBM_POISSON_FUSED ? "fused" : "split";
+}
+
+} // namespace
+
+void InamuroHIP::printPerformanceMetrics(std::ostream& os) const
+{
+    const lbm_perf_t p = perf();
+    if (p.steps <= 0) {
+        os << "performance: no timed steps yet" << std::endl;
+        return;
+    }
+    const double steps = static_cast<double>(p.steps);
+    const auto on = [](bool b) { return b ? "on" : "off"; };
+    os << "---- MI355X performance over " << p.steps << " timed steps ----\n"
+       << "precision " << (precision_ == LBM_PRECISION_MIXED ? "mixed" : "fp64") << ", Poisson "
+       << variant_name(accel_.variant) << ", pressure boundary " << (accel_.fused_boundary ? "fused" : "split")
+       << ", graph replay " << on(graph_) << "\n"
+       << "convergence: check every " << interval_ << " iterations, tolerance " << tol_ << ", iteration limit "
+       << limit_ << (limit_ == 0 ? " (unbounded)" : "") << "\n"
+       << "kinetic fixed-point residual: " << (dual_ == 1 ? "dual exit" : dual_ == 2 ? "evaluated" : "off")
+       << "; reductions: fixed-shape trees (deterministic)\n"
+       << "accelerators: relaxation scale " << accel_.pressure_relax_scale << ", fixed-point relaxation "
+       << accel_.fixed_point_relax << ", Anderson(1) " << on(accel_.anderson_m1 != 0) << " (beta max "
+       << accel_.anderson_beta_max << "), two-grid " << on(accel_.two_grid != 0) << " (strength "
+       << accel_.two_grid_strength << "), source-aware start " << on(accel_.source_aware_hh_init != 0)
+       << " (scale " << accel_.source_aware_hh_scale << "), scalar source scale " << accel_.scalar_source_scale
+       << ", spatial diagnostics " << on(accel_.spatial_diagnostics != 0) << "\n";
+    const double total = p.collide_ms + p.macro_ms + p.poisson_ms + p.check_ms + p.correct_ms;
+    os << "ms per step: collide+stream " << p.collide_ms / steps << ", macro " << p.macro_ms / steps
+       << ", Poisson iterations " << p.poisson_ms / steps << ", checks " << p.check_ms / steps << ", correct "
+       << p.correct_ms / steps << ", total " << total / steps << "\n"
+       << "Poisson iterations per step " << static_cast<double>(p.poisson_iterations) / steps << ", MLUPS "
+       << static_cast<double>(n) / (total / steps * 1000.0) << "\n";
+    if (detail_ && p.poisson_iter_launches > 0 && p.poisson_checks > 0)
+        os << "Poisson detail: " << p.poisson_ms * 1000.0 / static_cast<double>(p.poisson_iter_launches)
+           << " us per iteration, " << p.check_ms * 1000.0 / static_cast<double>(p.poisson_checks)
+           << " us per convergence check (" << static_cast<double>(p.poisson_checks) / steps << " per step)\n";
+    if (total > 0.0)
+        os << "share of step time: collide+stream " << 100.0 * p.collide_ms / total << "%, macro "
+           << 100.0 * p.macro_ms / total << "%, Poisson " << 100.0 * p.poisson_ms / total << "%, checks "
+           << 100.0 * p.check_ms / total << "%, correct " << 100.0 * p.correct_ms / total << "%\n";
+    os << std::flush;
+}
+
+void InamuroHIP::printRooflineSummary(std::ostream& os) const
+{
+    // Algorithmic bytes per cell: the step outside the solve (f and g read
+    // and written once, the macro fields, the source, the correction), one
+    // Poisson iteration (h read and written, p, tau_h and the source; the
+    // scalar sweep moves p, rho and the source only; an Anderson(1)
+    // iteration also reads the previous residual in its map and reads three
+    // and writes two distributions to mix), and one convergence check.
+    const bool mixed = precision_ == LBM_PRECISION_MIXED;
+    const double step_b = mixed ? 320.0 : 632.0;
+    double iter_b = mixed ? 144.0 : 272.0;
+    if (accel_.variant == LBM_POISSON_SCALAR)
+        iter_b = 32.0;
+    else if (accel_.variant == LBM_POISSON_ONEPASS && accel_.anderson_m1)
+        iter_b = 992.0;
+    const double cells = static_cast<double>(n);
+    os << "---- byte model, " << n << " cells ----\n"
+       << "step outside the Poisson solve: " << static_cast<int>(step_b) << " B/cell (" << step_b * cells / 1e9
+       << " GB)\n"
+       << "Poisson iteration: " << static_cast<int>(iter_b) << " B/cell (" << iter_b * cells / 1e9 << " GB)\n"
+       << "convergence check: 24 B/cell\n";
+    const lbm_perf_t p = perf();
+    if (p.poisson_iter_launches > 0 && p.poisson_ms > 0.0) {
+        const double ms = p.poisson_ms / static_cast<double>(p.poisson_iter_launches);
+        const double gbs = iter_b * cells / (ms * 1e-3) / 1e9;
+        os << "measured Poisson iteration: " << ms << " ms, " << gbs << " GB/s = " << 100.0 * gbs / 8000.0
+           << "% of 8 TB/s\n";
+    }
+    os << std::flush;
 }
 
 // The whole accelerator set goes through the library, which validates it

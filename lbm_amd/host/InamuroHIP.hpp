@@ -12,6 +12,7 @@
 
 #include <cstdint>
 #include <fstream>
+#include <iostream>
 #include <limits>
 #include <memory>
 #include <string>
@@ -29,13 +30,18 @@ public:
     InamuroHIP(const InamuroHIP&) = delete;
     InamuroHIP& operator=(const InamuroHIP&) = delete;
     InamuroHIP(InamuroHIP&& o) noexcept
-        : ctx(o.ctx), n(o.n), interval_(o.interval_), tol_(o.tol_), limit_(o.limit_), dual_(o.dual_),
+        : ctx(o.ctx), n(o.n), precision_(o.precision_), interval_(o.interval_), tol_(o.tol_), limit_(o.limit_),
+          dual_(o.dual_), deterministic_(o.deterministic_), detail_(o.detail_), graph_(o.graph_),
           accel_(o.accel_), fused_(o.fused_), onepass_(o.onepass_), scalar_(o.scalar_),
           pdiag_(std::move(o.pdiag_))
     {
         o.ctx = nullptr;
     }
 
+    // performTimeStepGPU refuses the dual-residual exit outside its contract
+    // (check interval 100, tolerance 1e-3, no iteration limit, deterministic
+    // reductions, no spatial diagnostics) with the reference's message
+    // (GPU/InamuroCUDA.cu:1857-1865); the C ABI itself allows any interval.
     void performTimeStepGPU();
     void synchronize() const;
     void setPoissonConvergence(int check_interval, double tolerance);
@@ -45,6 +51,22 @@ public:
     // Segment replay as a captured HIP graph (GPU/InamuroCUDA.hpp:80); on by
     // default, results identical either way.
     void setUsePoissonGraph(bool enabled);
+    // setEnablePoissonDetailTiming (GPU/InamuroCUDA.cu:2558-2561): the
+    // reference times each Poisson sub-kernel; here an iteration is one fused
+    // kernel, so the detail is the iteration kernel and the check per
+    // iteration and per step (printPerformanceMetrics).  Turns event timing on.
+    void setEnablePoissonDetailTiming(bool enabled);
+    // setUsePoissonDeterministicReductions (:2563-2566).  Every reduction of
+    // this build is a fixed-shape tree and deterministic whatever the flag;
+    // the flag is kept for the dual-residual contract check and the report.
+    // setUsePoissonDualResidual(true) sets it, as the reference's does.
+    void setUsePoissonDeterministicReductions(bool enabled);
+    // printPerformanceMetrics / printRooflineSummary / resetPerformanceMetrics
+    // (:2684-2830): the settings and the event-timed phase totals of
+    // lbm_perf_get, and this build's byte model per cell.
+    void printPerformanceMetrics(std::ostream& os = std::cout) const;
+    void printRooflineSummary(std::ostream& os = std::cout) const;
+    void resetPerformanceMetrics();
     // The optional Poisson accelerators (GPU/InamuroCUDA.hpp:60-76, 88),
     // FP64 only; each call validates as the reference's setter does and
     // installs the whole set through lbm_set_poisson_accel.  The three variant
@@ -87,10 +109,14 @@ private:
     void applyAccel(const lbm_poisson_accel_t& next);
     void* ctx = nullptr;
     std::size_t n = 0;
+    int precision_ = LBM_PRECISION_FP64;
     int interval_ = 100;
     double tol_ = 1.0e-3;
     std::uint64_t limit_ = 0;
     int dual_ = 0;
+    bool deterministic_ = false;  // the reference's default (GPU/InamuroCUDA.hpp:146)
+    bool detail_ = false;
+    bool graph_ = true;           // the library's default
     lbm_poisson_accel_t accel_{};
     bool fused_ = false, onepass_ = false, scalar_ = false;
     std::unique_ptr<std::ofstream> pdiag_;  // Poisson diagnostics CSV (heap: stable for the callback)

@@ -50,6 +50,8 @@ struct RunConfig {
     bool print_roofline = true;
     bool poisson_fixed_point_shadow = false;
     bool poisson_graph = false;
+    bool poisson_detail = false;
+    bool poisson_deterministic_reductions = false;
     bool poisson_dual_residual = false;
     std::string poisson_diagnostics;
     std::string dump_fields;
@@ -120,13 +122,14 @@ RunConfig parse_args(int argc, char** argv)
         else if (arg == "--output-frequency") { cfg.output_frequency = std::stoi(value()); freq_set = true; }
         else if (arg == "--output-steps") cfg.output_steps = parse_output_steps(value());
         else if (arg == "--write-output") cfg.write_output = true;
-        else if (arg == "--poisson-detail") {
-        } else if (arg == "--poisson-graph") cfg.poisson_graph = true;
-        else if (arg == "--poisson-deterministic-reductions") {
-        } else if (arg == "--poisson-fixed-point-shadow") cfg.poisson_fixed_point_shadow = true;
+        else if (arg == "--poisson-detail") cfg.poisson_detail = true;
+        else if (arg == "--poisson-graph") cfg.poisson_graph = true;
+        else if (arg == "--poisson-deterministic-reductions") cfg.poisson_deterministic_reductions = true;
+        else if (arg == "--poisson-fixed-point-shadow") cfg.poisson_fixed_point_shadow = true;
         else if (arg == "--poisson-dual-residual") {
             cfg.poisson_dual_residual = true;
             cfg.poisson_fixed_point_shadow = true;
+            cfg.poisson_deterministic_reductions = true;  // GPU/main.cpp:186-191
         } else if (arg == "--poisson-diagnostics") cfg.poisson_diagnostics = value();
         else if (arg == "--poisson-spatial-diagnostics") cfg.poisson_spatial_diagnostics = true;
         else if (arg == "--no-roofline") cfg.print_roofline = false;
@@ -213,6 +216,8 @@ int main(int argc, char** argv)
         lbm::Inamuro solver(P);
         lbm::InamuroHIP& gpu = solver.gpu();
         gpu.setTiming(true);
+        gpu.setEnablePoissonDetailTiming(cfg.poisson_detail);
+        gpu.setUsePoissonDeterministicReductions(cfg.poisson_deterministic_reductions);
         gpu.setPoissonConvergence(cfg.poisson_check_interval, cfg.poisson_tolerance);
         gpu.setPoissonIterationLimit(cfg.poisson_iteration_limit);
         gpu.setEnablePoissonFixedPointShadow(cfg.poisson_fixed_point_shadow);
@@ -253,30 +258,9 @@ int main(int argc, char** argv)
         std::cout << std::fixed << std::setprecision(3) << "GPU wall_total_ms=" << ms
                   << " wall_avg_ms_per_step=" << ms / std::max(cfg.steps, 1)
                   << " wall_MLUPS=" << cells * cfg.steps / (ms * 1000.0) << std::endl;
-        const lbm_perf_t pf = gpu.perf();
-        if (pf.steps > 0) {
-            const double iter_ms = pf.poisson_ms / std::max<double>(1.0, static_cast<double>(pf.poisson_iter_launches));
-            std::cout << "Average per step (ms): collide " << pf.collide_ms / pf.steps << ", macro "
-                      << pf.macro_ms / pf.steps << ", Poisson " << pf.poisson_ms / pf.steps << ", checks "
-                      << pf.check_ms / pf.steps << ", correct " << pf.correct_ms / pf.steps
-                      << "; Poisson iterations/step " << static_cast<double>(pf.poisson_iterations) / pf.steps
-                      << ", ms/iteration " << iter_ms << std::endl;
-            if (cfg.print_roofline) {
-                // Algorithmic bytes per cell of one iteration: h read and
-                // written, p read and written, tau_h and the source read;
-                // scalar Jacobi moves p, rho and the source only; an
-                // Anderson(1) iteration also reads the previous residual in
-                // the map and reads three / writes two distributions to mix.
-                double bpc = P.precision == "mixed" ? 144.0 : 272.0;
-                if (cfg.poisson == "scalar")
-                    bpc = 32.0;
-                else if (cfg.poisson == "onepass" && cfg.poisson_anderson_m1)
-                    bpc = 992.0;
-                std::cout << "Poisson iteration: " << bpc << " algorithmic bytes/cell, "
-                          << bpc * cells / (iter_ms * 1e-3) / 1e9 << " GB/s = "
-                          << 100.0 * bpc * cells / (iter_ms * 1e-3) / 8.0e12 << "% of 8 TB/s" << std::endl;
-            }
-        }
+        // GPU/main.cpp:373-376
+        gpu.printPerformanceMetrics();
+        if (cfg.print_roofline) gpu.printRooflineSummary();
         if (!cfg.dump_fields.empty()) {
             std::vector<double> f[6];
             gpu.downloadFields(f);

@@ -32,6 +32,9 @@ def test_runner_matches_oracle_with_gpu_tree_initialisation(tmp_path):
                         "--poisson-diagnostics", str(diag)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     assert "wall_MLUPS=" in r.stdout
+    # printPerformanceMetrics and printRooflineSummary (GPU/main.cpp:373-376)
+    assert "---- MI355X performance over 3 timed steps ----" in r.stdout
+    assert "Poisson iteration: 272 B/cell" in r.stdout and "% of 8 TB/s" in r.stdout
     # GPU-tree default: two sharp droplets through place_droplet at x = 0.
     o = po.Oracle(P)
     o.place_droplet(0.0, P.ly / 2.0, P.lz / 2.0 + P.DD, vz=-0.035, profile="sharp")
@@ -149,3 +152,62 @@ def test_solver_writes_reference_step_diagnostics(tmp_path):
         assert data[:8] == b"#!TDV101"
         assert len(data) > P.lx * P.ly * P.lz * 36
     assert not (out / "steps.csv.lock").exists()
+
+
+WRAPPER_DRIVER = r"""
+#include <iostream>
+#include <stdexcept>
+#include "InamuroHIP.hpp"
+int main()
+{
+    lbm_params_t P{16, 32, 32, 50.0, 1.0, 0.008, 0.00016, 1.0, 1.0, 0.5, 0.0005, 0.035, 1.0, 6.7,
+                   0.09714, 0.01134, 0.092, 0.015, 8.0};
+    lbm::InamuroHIP gpu(P, LBM_PRECISION_FP64);
+    gpu.initializeReferenceDroplets();
+    gpu.setPoissonConvergence(50, 1e-3);
+    gpu.setUsePoissonDualResidual(true);
+    try { gpu.performTimeStepGPU(); return 2; } catch (const std::runtime_error& e) { std::cout << "A " << e.what() << std::endl; }
+    gpu.setPoissonConvergence(100, 1e-3);
+    gpu.setUsePoissonDeterministicReductions(false);
+    try { gpu.performTimeStepGPU(); return 3; } catch (const std::runtime_error& e) { std::cout << "B " << e.what() << std::endl; }
+    gpu.setUsePoissonDeterministicReductions(true);
+    gpu.setEnablePoissonDetailTiming(true);
+    gpu.performTimeStepGPU();
+    gpu.performTimeStepGPU();
+    std::cout << "C " << gpu.getLastPoissonDiagnostics().fixed_point_evaluated << std::endl;
+    gpu.printPerformanceMetrics();
+    gpu.printRooflineSummary();
+    gpu.resetPerformanceMetrics();
+    gpu.printPerformanceMetrics();
+    return 0;
+}
+"""
+
+
+def test_wrapper_mirrors_the_dual_residual_contract_and_reports(tmp_path):
+    """lbm::InamuroHIP refuses the dual-residual exit outside its contract with
+    the reference's message (GPU/InamuroCUDA.cu:1857-1865): a check interval
+    of 50, then deterministic reductions switched off after the dual residual
+    switched them on (:2573-2580).  Within the contract it steps, and the
+    reports carry the detail timing and the byte model."""
+    src = tmp_path / "driver.cpp"
+    src.write_text(WRAPPER_DRIVER)
+    exe = tmp_path / "driver"
+    host = os.path.join(ROOT, "lbm_amd", "host")
+    cmd = ["g++", "-std=c++17", "-O1", f"-I{ROOT}/include", f"-I{host}", str(src),
+           os.path.join(host, "InamuroHIP.cpp"), "-o", str(exe), f"-L{ROOT}/lbm_amd", "-llbm_hip",
+           f"-Wl,-rpath,{ROOT}/lbm_amd", "-Wl,-rpath,/opt/rocm/lib"]
+    b = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+    assert b.returncode == 0, b.stderr
+    print("wrapper driver built; running it", flush=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, (r.returncode, r.stdout, r.stderr)
+    msg = ("dual-residual exit requires check_interval=100, tolerance=1e-3, poisson_iteration_limit=0, "
+           "deterministic reductions, no spatial diagnostics")
+    assert "A " + msg in r.stdout and "B " + msg in r.stdout
+    assert "C 1" in r.stdout
+    assert "---- MI355X performance over 2 timed steps ----" in r.stdout
+    assert "kinetic fixed-point residual: dual exit" in r.stdout
+    assert "Poisson detail: " in r.stdout and " us per iteration" in r.stdout
+    assert "Poisson iteration: 272 B/cell" in r.stdout and "measured Poisson iteration:" in r.stdout
+    assert r.stdout.rstrip().endswith("performance: no timed steps yet")
