@@ -25,7 +25,8 @@ Other legs: --workload head_on_collision | falling_droplet | static_droplet_64
 --adjoint-falling T (the falling droplet's centroid-height adjoint at full
 size); --traffic-probe (one step per precision, for counter runs); --accel
 (the reference's Poisson accelerators: iterations to tolerance and time per
-iteration).
+iteration); --mixed-drift N (mixed against FP64 in lockstep over N steps of
+--workload).
 """
 from __future__ import annotations
 
@@ -364,6 +365,56 @@ ACCEL_VARIANTS = [
 ]
 
 
+def run_mixed_drift(hip, workload, steps):
+    """Mixed precision against FP64 in lockstep on one workload: the Poisson
+    iteration counts of every step and, every 10 steps (and after steps 1-3),
+    the relative L-inf error of every field and the spurious-current
+    magnitude max|u| of the FP64 run."""
+    import numpy as np
+
+    wl = WORKLOADS[workload]
+    f64, P = make_sim(hip, wl, "fp64")
+    mx, _ = make_sim(hip, wl, "mixed")
+    # control: FP64 from the start state rounded to FP32, i.e. how far the
+    # FP64 trajectory itself carries differences of FP32 size
+    start = f64.downloadFieldsToCPU()
+    ctl = hip.InamuroHIP(P, precision=hip.FP64)
+    ctl.setUsePoissonGraph(GRAPHS)
+    if wl.get("zwalls"):
+        ctl.set_z_walls(True)
+    if wl.get("gz"):
+        ctl.set_gravity(wl["gz"])
+    ctl.upload_fields({k: v.astype(np.float32).astype(np.float64) for k, v in start.items()})
+    del start
+    fields = ("fei", "rho", "u", "v", "w", "p")
+    rows, counts_equal = [], True
+    worst = {"mixed": dict.fromkeys(fields, 0.0), "fp64_from_fp32_start": dict.fromkeys(fields, 0.0)}
+    t0 = time.perf_counter()
+
+    def rel(B, A):
+        return {k: float(np.abs(B[k] - A[k]).max()) / max(float(np.abs(A[k]).max()), 1e-300) for k in fields}
+
+    for s in range(1, steps + 1):
+        a, b, c = f64.step(1)[0], mx.step(1)[0], ctl.step(1)[0]
+        counts_equal &= a == b
+        msg = f"mixed drift: step {s}, iterations fp64 {a} mixed {b} control {c}"
+        if s % 10 == 0 or s <= 3 or s == steps:
+            A = f64.downloadFieldsToCPU()
+            err = {"mixed": rel(mx.downloadFieldsToCPU(), A), "fp64_from_fp32_start": rel(ctl.downloadFieldsToCPU(), A)}
+            for run, e in err.items():
+                for k in fields:
+                    worst[run][k] = max(worst[run][k], e[k])
+            rows.append(dict(step=s, iterations=a, control_iterations=c, linf_rel=err,
+                             max_speed_fp64=float(np.abs(A["u"]).max())))
+            msg += " | mixed " + " ".join(f"{k} {v:.2e}" for k, v in err["mixed"].items())
+            msg += " | control u " + f"{max(err['fp64_from_fp32_start'][k] for k in ('u', 'v', 'w')):.2e}"
+        print(f"{msg} [{time.perf_counter() - t0:.0f} s]", file=sys.stderr, flush=True)
+    for sim in (f64, mx, ctl):
+        sim.close()
+    return {"mixed_drift": dict(workload=workload, steps=steps, iteration_counts_equal=bool(counts_equal),
+                                worst_linf_rel=worst, samples=rows)}
+
+
 def run_accel(hip, steps=3, bw_iterations=1000):
     """Convergence and bandwidth of each accelerator.  Convergence: the 64^3
     static droplet (FP64), `steps` steps from the cold start, iterations per
@@ -530,6 +581,9 @@ def main(argv=None):
     ap.add_argument("--accel", action="store_true",
                     help="the reference's Poisson accelerators (FP64): iterations to tolerance on the 64^3 "
                          "droplet and time per iteration at 256^3; prints one JSON line and exits")
+    ap.add_argument("--mixed-drift", type=int, default=0, metavar="N",
+                    help="mixed precision against FP64 in lockstep over N steps of --workload (iteration counts "
+                         "and field errors every 10 steps); prints one JSON line and exits")
     ap.add_argument("--accel-probe", action="store_true",
                     help="the pull-form accelerators (onepass, onepass + Anderson(1), scalar) at 256^3 with the "
                          "Poisson counts frozen at [100, 300] (for rocprofv3 runs) and exit")
@@ -551,6 +605,9 @@ def main(argv=None):
 
     if args.accel:
         print(json.dumps(run_accel(hip)), flush=True)
+        return 0
+    if args.mixed_drift:
+        print(json.dumps(run_mixed_drift(hip, args.workload, args.mixed_drift)), flush=True)
         return 0
     if args.accel_probe:
         for name, accel, _ in ACCEL_VARIANTS:
