@@ -37,6 +37,31 @@ constexpr float FWF[Q] = {-7.0f / 3.0f,
                           1.0f / 24.0f, 1.0f / 24.0f, 1.0f / 24.0f, 1.0f / 24.0f,
                           1.0f / 24.0f, 1.0f / 24.0f, 1.0f / 24.0f, 1.0f / 24.0f};
 
+// Products with a lattice weight.  Rounded to FP32 the weights are all off
+// the same way (2/9, 1/9 and 1/72 up by 7.5e-9 relative, 1/3 and 1/24 up by
+// 3e-8, -7/3 down by 3.4e-8), so w x with the rounded weight carries one
+// relative bias in every direction of every cell, and the moments of the
+// equilibria (the momentum of g, the phi flux of f) gain that factor on
+// every step; the bias then accumulates over a run instead of averaging
+// out.  Each weight is kept as the pair hi + lo (hi the FP32 weight, lo its
+// rounding error) and multiplied as fma(hi, x, lo x): exact but for the
+// last rounding, which falls either way.
+constexpr float lo_part(double w) { return static_cast<float>(w - static_cast<double>(static_cast<float>(w))); }
+constexpr float EWF_LO[Q] = {lo_part(2.0 / 9.0),
+                             lo_part(1.0 / 9.0), lo_part(1.0 / 9.0), lo_part(1.0 / 9.0),
+                             lo_part(1.0 / 9.0), lo_part(1.0 / 9.0), lo_part(1.0 / 9.0),
+                             lo_part(1.0 / 72.0), lo_part(1.0 / 72.0), lo_part(1.0 / 72.0), lo_part(1.0 / 72.0),
+                             lo_part(1.0 / 72.0), lo_part(1.0 / 72.0), lo_part(1.0 / 72.0), lo_part(1.0 / 72.0)};
+constexpr float FWF_LO[Q] = {lo_part(-7.0 / 3.0),
+                             lo_part(1.0 / 3.0), lo_part(1.0 / 3.0), lo_part(1.0 / 3.0),
+                             lo_part(1.0 / 3.0), lo_part(1.0 / 3.0), lo_part(1.0 / 3.0),
+                             lo_part(1.0 / 24.0), lo_part(1.0 / 24.0), lo_part(1.0 / 24.0), lo_part(1.0 / 24.0),
+                             lo_part(1.0 / 24.0), lo_part(1.0 / 24.0), lo_part(1.0 / 24.0), lo_part(1.0 / 24.0)};
+
+// E_q x and F_q x
+__device__ __forceinline__ float ew(int q, float x) { return fmaf(EWF[q], x, EWF_LO[q] * x); }
+__device__ __forceinline__ float fw(int q, float x) { return fmaf(FWF[q], x, FWF_LO[q] * x); }
+
 // c_a * x for a compile-time lattice component c_a in {-1, 0, 1}.
 template <int C>
 __device__ __forceinline__ float cm(float x)
@@ -152,22 +177,22 @@ __device__ __forceinline__ void collide_dir(const PhysF& ph, float feic, float u
     const float Gf = 4.5f * fe * fe - 1.5f * sum_fei * ci_sq;
     const float Gr = 4.5f * re * re - 1.5f * sum_rho * ci_sq;
     // f_eq without the H_q * phi term (it is the stored base of f'_0).
-    const float feq = FWF[I] * P0p + 3.0f * EWF[I] * feic * un + EWF[I] * ph.k_f * Gf;
+    const float feq = fw(I, P0p) + ew(I, 3.0f * feic * un + ph.k_f * Gf);
     // sum_ab c_a c_b d_b u_a for this direction.
     float cc = 0.f;
     if (CX[I] != 0) cc += cm<CX[I]>(cm<CX[I]>(U.gx) + cm<CY[I]>(U.gy) + cm<CZ[I]>(U.gz));
     if (CY[I] != 0) cc += cm<CY[I]>(cm<CX[I]>(V.gx) + cm<CY[I]>(V.gy) + cm<CZ[I]>(V.gz));
     if (CZ[I] != 0) cc += cm<CZ[I]>(cm<CX[I]>(W.gx) + cm<CY[I]>(W.gy) + cm<CZ[I]>(W.gz));
     // g_eq - E_q.
-    const float geq = EWF[I] * (3.0f * un - 1.5f * usq + 4.5f * un * un + ph.visc * cc) +
-                      EWF[I] * kg_rho * Gr - (2.0f / 3.0f) * FWF[I] * kg_rho * sum_rho;
+    const float geq = ew(I, 3.0f * un - 1.5f * usq + 4.5f * un * un + ph.visc * cc + kg_rho * Gr) -
+                      fw(I, (2.0f / 3.0f) * kg_rho * sum_rho);
     if (MODE == FG_FIRST) {
         f = feq;
         g = geq;
     } else {
         const float lapc = dot3<I>(U.lap, V.lap, W.lap);
         f = f - (f - feq) * ph.tauf_inv;
-        g = g - (g - geq) * ph.taug_inv + 3.0f * EWF[I] * force_scale * lapc;
+        g = g - (g - geq) * ph.taug_inv + ew(I, 3.0f * force_scale * lapc);
     }
 }
 
@@ -261,7 +286,7 @@ __global__ __launch_bounds__(BLOCK) void k_collide(
                 const float az = ph.gz * (1.0f - ph.rho_G * inv_rho);
 #pragma unroll
                 for (int q = 0; q < Q; ++q)
-                    if (CZ[q] != 0) gv[q] += 3.0f * EWF[q] * static_cast<float>(CZ[q]) * az;
+                    if (CZ[q] != 0) gv[q] += ew(q, 3.0f * static_cast<float>(CZ[q]) * az);
             }
 #pragma unroll
             for (int q = 0; q < Q; ++q) {
@@ -449,7 +474,7 @@ __global__ __launch_bounds__(PB) void k_poisson_push(
     for (int q = 0; q < Q; ++q) {
         // h* - E_q b = (h - E_q p)(1 - 1/tau_h) - E_q S/3 + E_q (p - b), with
         // d (1 - 1/tau_h) = d a - d formed in one rounding (a = 2 - 1/tau_h)
-        const float o = fmaf(dev[q], om, -dev[q]) - EWF[q] * s3 + static_cast<float>(EW[q] * rb);
+        const float o = fmaf(dev[q], om, -dev[q]) - ew(q, s3) + static_cast<float>(EW[q] * rb);
         const int dx = CX[q] == 1 ? xp : (CX[q] == -1 ? xm : x);
         const int dy = CY[q] == 1 ? yp : (CY[q] == -1 ? ym : y);
         const int dz = CZ[q] == 1 ? zp : (CZ[q] == -1 ? zm : z);
