@@ -134,12 +134,12 @@ void InamuroHIP::printPerformanceMetrics(std::ostream& os) const
 {
     const lbm_perf_t p = perf();
     if (p.steps <= 0) {
-        os << "performance: no timed steps yet" << std::endl;
+        os << "performance: no steps since the last reset" << std::endl;
         return;
     }
     const double steps = static_cast<double>(p.steps);
     const auto on = [](bool b) { return b ? "on" : "off"; };
-    os << "---- MI355X performance over " << p.steps << " timed steps ----\n"
+    os << "---- MI355X performance over " << p.steps << " steps ----\n"
        << "precision " << (precision_ == LBM_PRECISION_MIXED ? "mixed" : "fp64") << ", Poisson "
        << variant_name(accel_.variant) << ", pressure boundary " << (accel_.fused_boundary ? "fused" : "split")
        << ", graph replay " << on(graph_) << "\n"
@@ -154,6 +154,12 @@ void InamuroHIP::printPerformanceMetrics(std::ostream& os) const
        << " (scale " << accel_.source_aware_hh_scale << "), scalar source scale " << accel_.scalar_source_scale
        << ", spatial diagnostics " << on(accel_.spatial_diagnostics != 0) << "\n";
     const double total = p.collide_ms + p.macro_ms + p.poisson_ms + p.check_ms + p.correct_ms;
+    if (!(total > 0.0)) {
+        // the steps ran with event timing off (setTiming / setEnablePoissonDetailTiming)
+        os << "Poisson iterations per step " << static_cast<double>(p.poisson_iterations) / steps
+           << "; no phase times (event timing was off)" << std::endl;
+        return;
+    }
     os << "ms per step: collide+stream " << p.collide_ms / steps << ", macro " << p.macro_ms / steps
        << ", Poisson iterations " << p.poisson_ms / steps << ", checks " << p.check_ms / steps << ", correct "
        << p.correct_ms / steps << ", total " << total / steps << "\n"
