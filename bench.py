@@ -366,10 +366,11 @@ ACCEL_VARIANTS = [
 
 
 def run_mixed_drift(hip, workload, steps):
-    """Mixed precision against FP64 in lockstep on one workload: the Poisson
+    """Mixed precision against FP64 in lockstep on one workload, beside a
+    control (FP64 started from the FP32-rounded start state): the Poisson
     iteration counts of every step and, every 10 steps (and after steps 1-3),
-    the relative L-inf error of every field and the spurious-current
-    magnitude max|u| of the FP64 run."""
+    the relative L-inf error of every field of both against FP64 and the
+    spurious-current magnitude max|u| of the FP64 run."""
     import numpy as np
 
     wl = WORKLOADS[workload]
