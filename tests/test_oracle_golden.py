@@ -83,3 +83,25 @@ def test_tanh_initialisation_matches_gpu_tree_constructor(tmp_path):
     o.place_droplet(cx, cy, cz, vz=vz, profile="tanh", width=wdt)
     for k in po.FIELD_NAMES:
         assert np.array_equal(o.get(k).view(np.uint64), ref[k].view(np.uint64)), k
+
+
+@pytest.mark.skipif(not po.ref_available(), reason="oracle/_ref is built only where the reference tree exists")
+@pytest.mark.parametrize("shape,dd,centre,steps", [((3, 66, 21), 2.0, (1.5, 33.0, 10.0), 2),
+                                                   ((70, 9, 13), 6.0, (35.0, 4.5, 6.5), 1)])
+def test_oracle_matches_live_reference_on_ragged_grids(tmp_path, shape, dd, centre, steps):
+    """The grids of the GPU's partial-tile tests (tests/test_gpu_ragged.py),
+    with a moving tanh droplet uploaded into the reference: every field and
+    population bit for bit, the same iteration counts (3x66x21: 2000, 2300;
+    70x9x13: 7500)."""
+    lx, ly, lz = shape
+    P = po.Params(lx=lx, ly=ly, lz=lz, DD=dd)
+    o = po.Oracle(P)
+    o.place_droplet(*centre, 0.01, -0.005, 0.002, profile="tanh", width=2.0)
+    pfile, init, out = tmp_path / "p.in", tmp_path / "init.bin", tmp_path / "out.bin"
+    pfile.write_text(P.params_in())
+    po.write_init(str(init), o.fields())
+    subprocess.run([po.REF_PROBE, str(pfile), str(init), str(steps), str(out), "1"], check=True)
+    iters, _, ref = po.read_probe_output(str(out))
+    assert o.step(steps) == list(map(int, iters))
+    for k in list(po.FIELD_NAMES) + ["f", "g", "h"]:
+        assert np.array_equal(o.get(k).view(np.uint64), ref[k].view(np.uint64)), k
