@@ -13,6 +13,18 @@
  * Extensions that the reference does not have (dual residual on the CPU,
  * tolerance / interval parameters, iteration watchdog, droplet velocity
  * vectors) are marked as such.
+ *
+ * How each part is pinned (none is left unpinned):
+ *   the step, its stages and the initialisers -- bit for bit against the
+ *     compiled reference (tests/test_oracle_golden.py, golden fixtures);
+ *   the dual residual (a CUDA-only path of the reference) -- a closed form
+ *     and a rebuild from the pinned stages (tests/test_oracle_dual_residual.py);
+ *   the Poisson accelerators (CUDA-only) -- equivalences with the strict
+ *     loop, closed forms and rebuilds (tests/test_oracle_accel.py);
+ *   z walls and gravity (no reference counterpart) -- closed forms
+ *     (tests/test_oracle_walls.py);
+ *   the adjoint and the objectives -- central differences of the oracle's
+ *     own forward map (tests/test_oracle_adjoint.py).
  */
 #include "lbm_oracle.h"
 
